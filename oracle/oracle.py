@@ -1,0 +1,345 @@
+"""ctypes binding of the CPU f64 oracle (test infrastructure only).
+
+The oracle restates Gravlume's reference CPU trace (crates/gravlume-reference,
+crates/gravlume-domain) in C++ and is the checker for the HIP trace in
+gravlume_amd. Only tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg may import this module; the product never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+
+TERM_HORIZON = 1
+TERM_ESCAPE = 2
+TERM_SINGULARITY = 3
+TERM_STEP_EXHAUSTION = 4
+TERM_NUMERICAL_FAILURE = 5
+TERM_SURFACE = 7
+TERM_REJECT_EXHAUSTION = 8
+
+
+class Spacetime(C.Structure):
+    _fields_ = [("mass", C.c_double), ("spin", C.c_double), ("charge", C.c_double),
+                ("chart", C.c_double), ("extremality", C.c_int), ("has_horizon", C.c_bool),
+                ("outer_horizon", C.c_double)]
+
+
+class Policy(C.Structure):
+    _fields_ = [("pos_rtol", C.c_double), ("pos_atol", C.c_double), ("mom_rtol", C.c_double),
+                ("mom_atol", C.c_double), ("initial_step", C.c_double),
+                ("minimum_step", C.c_double), ("maximum_step", C.c_double),
+                ("safety", C.c_double), ("min_factor", C.c_double), ("max_factor", C.c_double),
+                ("max_accepted", C.c_uint64), ("max_rejects", C.c_uint32),
+                ("event_affine_tol", C.c_double), ("event_tie_tol", C.c_double),
+                ("arming_band", C.c_double), ("guard_d_over_m4", C.c_double), ("id", C.c_int)]
+
+
+class Events(C.Structure):
+    _fields_ = [("has_escape", C.c_int), ("escape_radius", C.c_double),
+                ("has_surface", C.c_int), ("surface_inner", C.c_double),
+                ("surface_outer", C.c_double), ("horizon_offset", C.c_double)]
+
+
+class Emitter(C.Structure):
+    _fields_ = [("intensity_at_six", C.c_double), ("blackbody", C.c_int),
+                ("temperature_at_six", C.c_double)]
+
+
+class Outcome(C.Structure):
+    _fields_ = [
+        ("termination", C.c_int), ("failure", C.c_int), ("candidates", C.c_uint32),
+        ("polar_side", C.c_int), ("radial_turnings", C.c_uint32),
+        ("equatorial_crossings", C.c_uint32), ("azimuth_winding", C.c_int32),
+        ("has_turning_radius", C.c_int), ("turning_radius", C.c_double),
+        ("azimuth_advance", C.c_double), ("travel_time", C.c_double), ("affine", C.c_double),
+        ("state", C.c_double * 8), ("accepted", C.c_uint64), ("rejected", C.c_uint64),
+        ("rhs_evaluations", C.c_uint64), ("min_step", C.c_double), ("max_step", C.c_double),
+        ("drift_null", C.c_double), ("drift_energy", C.c_double), ("drift_lz", C.c_double),
+        ("drift_carter", C.c_double), ("has_event", C.c_int), ("event_affine", C.c_double),
+        ("event_bracket", C.c_double), ("event_residual", C.c_double),
+        ("has_escape_direction", C.c_int), ("escape_direction", C.c_double * 3),
+        ("surface_status", C.c_int), ("source_radius", C.c_double),
+        ("source_azimuth", C.c_double), ("frequency_ratio", C.c_double),
+        ("emitted_intensity", C.c_double), ("observed_intensity", C.c_double),
+        ("emitted_temperature", C.c_double), ("observed_temperature", C.c_double),
+    ]
+
+
+class Observation(C.Structure):
+    _fields_ = [
+        ("st", Spacetime), ("event", C.c_double * 4), ("u", C.c_double * 4),
+        ("right", C.c_double * 4), ("up", C.c_double * 4), ("arrival", C.c_double * 4),
+        ("gram_residual", C.c_double), ("orientation", C.c_double),
+        ("used_up_fallback", C.c_int), ("g_tt", C.c_double), ("metric", (C.c_double * 4) * 4),
+        ("omega", C.c_double), ("width", C.c_uint32), ("height", C.c_uint32),
+        ("fov", C.c_double), ("tan_half_fov", C.c_double),
+    ]
+
+
+_lib = None
+
+
+def build() -> None:
+    """Compile the oracle with the committed Makefile (host g++ only)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        P = C.POINTER
+        dp = P(C.c_double)
+        L.orc_trace.argtypes = [P(Spacetime), P(Policy), P(Events), dp, C.c_double, P(Outcome)]
+        L.orc_trace_ordered.argtypes = [P(Spacetime), P(Policy), P(Events), dp, dp, C.c_size_t,
+                                        C.c_uint, P(Outcome)]
+        L.orc_observation_trace_ordered.argtypes = [P(Observation), P(Policy), P(Events),
+                                                    P(Emitter), dp, C.c_size_t, C.c_uint,
+                                                    P(Outcome), P(C.c_int)]
+        L.orc_build_observation.argtypes = [C.c_double] * 4 + [dp, dp, dp, C.c_double, C.c_uint32,
+                                                               C.c_uint32, C.c_double,
+                                                               P(Observation)]
+        L.orc_make_spacetime.argtypes = [C.c_double] * 4 + [P(Spacetime)]
+        L.orc_geometry.argtypes = [P(Spacetime), dp, dp, dp, dp, dp, dp, dp]
+        L.orc_metric.argtypes = [P(Spacetime), dp, dp, dp]
+        L.orc_rhs.argtypes = [P(Spacetime), dp, dp]
+        L.orc_invariants.argtypes = [P(Spacetime), dp, dp]
+        L.orc_radial_velocity.argtypes = [P(Spacetime), dp, dp]
+        L.orc_sight_plane.argtypes = [P(Observation), C.c_uint32, C.c_uint32, C.c_double,
+                                      C.c_double, dp]
+        L.orc_initial_ray.argtypes = [P(Observation), C.c_double, C.c_double, dp, dp, dp]
+        L.orc_observable_at.argtypes = [P(Spacetime), P(Emitter), C.c_double, C.c_double, dp,
+                                        C.c_double, P(Outcome)]
+        L.orc_oblate_to_cartesian.argtypes = [C.c_double] * 5 + [dp]
+        L.orc_isco_prograde.argtypes = [C.c_double, C.c_double]
+        L.orc_isco_prograde.restype = C.c_double
+        L.orc_compare_extremality.argtypes = [C.c_double] * 3
+        L.orc_policy_preset.argtypes = [C.c_int, P(Policy)]
+        for name, struct in (("outcome", Outcome), ("observation", Observation),
+                             ("policy", Policy), ("spacetime", Spacetime), ("events", Events)):
+            fn = getattr(L, f"orc_sizeof_{name}")
+            fn.restype = C.c_size_t
+            assert fn() == C.sizeof(struct), f"oracle struct layout mismatch: {name}"
+        _lib = L
+    return _lib
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def policy(preset: str = "regular", **overrides) -> Policy:
+    p = Policy()
+    rc = lib().orc_policy_preset({"regular": 0, "strict": 1}[preset], C.byref(p))
+    assert rc == 0
+    for k, v in overrides.items():
+        setattr(p, k, v)
+    if overrides:
+        p.id = 2
+    return p
+
+
+def scaled_policy(base: Policy, divisor: float) -> Policy:
+    """The same policy with every rtol/atol divided by `divisor`."""
+    p = Policy()
+    C.memmove(C.byref(p), C.byref(base), C.sizeof(Policy))
+    for k in ("pos_rtol", "pos_atol", "mom_rtol", "mom_atol"):
+        setattr(p, k, getattr(base, k) / divisor)
+    p.id = 2
+    return p
+
+
+def config_policy(rtol: float, atol: float, maximum_step: float = 0.5) -> Policy:
+    return policy("regular", pos_rtol=rtol, pos_atol=atol, mom_rtol=rtol, mom_atol=atol,
+                  maximum_step=maximum_step)
+
+
+def spacetime(mass=1.0, spin=0.0, charge=0.0, chart=1.0) -> Spacetime:
+    st = Spacetime()
+    rc = lib().orc_make_spacetime(mass, spin, charge, chart, C.byref(st))
+    if rc:
+        raise ValueError("invalid spacetime")
+    return st
+
+
+def events(escape_radius=200.0, surface=None, horizon_offset=0.0, has_escape=True) -> Events:
+    ev = Events()
+    ev.has_escape = 1 if (has_escape and escape_radius is not None) else 0
+    ev.escape_radius = escape_radius if escape_radius is not None else 0.0
+    if surface is not None:
+        ev.has_surface = 1
+        ev.surface_inner, ev.surface_outer = surface
+    ev.horizon_offset = horizon_offset
+    return ev
+
+
+def trace(st: Spacetime, pol: Policy, ev: Events, state, direction: float = 1.0) -> Outcome:
+    y = np.ascontiguousarray(state, dtype=np.float64)
+    out = Outcome()
+    lib().orc_trace(C.byref(st), C.byref(pol), C.byref(ev), _dp(y), direction, C.byref(out))
+    return out
+
+
+def trace_ordered(st, pol, ev, states, directions, threads: int = 1):
+    y = np.ascontiguousarray(states, dtype=np.float64).reshape(-1, 8)
+    d = np.ascontiguousarray(directions, dtype=np.float64).reshape(-1)
+    n = y.shape[0]
+    out = (Outcome * n)()
+    rc = lib().orc_trace_ordered(C.byref(st), C.byref(pol), C.byref(ev), _dp(y), _dp(d), n,
+                                 threads, out)
+    if rc:
+        raise ValueError("thread count must lie in [1, 256]")
+    return out
+
+
+def observation(mass, spin, charge, chart, event, target, up, omega, width, height, fov):
+    o = Observation()
+    e = np.ascontiguousarray(event, dtype=np.float64)
+    t = np.ascontiguousarray(target, dtype=np.float64)
+    u = np.ascontiguousarray(up, dtype=np.float64)
+    rc = lib().orc_build_observation(mass, spin, charge, chart, _dp(e), _dp(t), _dp(u), omega,
+                                     width, height, fov, C.byref(o))
+    if rc:
+        raise ValueError(f"observation rejected (code {rc})")
+    return o
+
+
+def oblate_to_cartesian(spin: float, chart: float, radius: float, polar: float, azimuth: float):
+    """KerrNewmanSpacetime::oblate_to_cartesian (crates/gravlume-domain/src/spacetime.rs:245-254)."""
+    out = np.zeros(3)
+    lib().orc_oblate_to_cartesian(spin, chart, radius, polar, azimuth, _dp(out))
+    return out
+
+
+def sight_plane(obs: Observation, px: int, py: int, subx=0.5, suby=0.5):
+    out = np.zeros(2)
+    rc = lib().orc_sight_plane(C.byref(obs), px, py, subx, suby, _dp(out))
+    if rc:
+        raise ValueError("sample outside the view")
+    return out
+
+
+def sight_planes(obs: Observation, xs, ys, subx=0.5, suby=0.5) -> np.ndarray:
+    """Vectorised PerspectiveView::sight_plane for pixel centres (view.rs:107-123)."""
+    xs = np.asarray(xs, dtype=np.float64)
+    ys = np.asarray(ys, dtype=np.float64)
+    w, h = float(obs.width), float(obs.height)
+    nx = 2.0 * (xs + subx) / w - 1.0
+    ny = 1.0 - 2.0 * (ys + suby) / h
+    return np.stack([(w / h) * obs.tan_half_fov * nx, obs.tan_half_fov * ny], axis=-1)
+
+
+def initial_ray(obs: Observation, sx: float, sy: float):
+    y = np.zeros(8)
+    sight = np.zeros(4)
+    fn = np.zeros(2)
+    rc = lib().orc_initial_ray(C.byref(obs), sx, sy, _dp(y), _dp(sight), _dp(fn))
+    if rc:
+        raise ValueError(f"initial ray rejected (code {rc})")
+    return y, sight, fn[0], fn[1]
+
+
+def observation_trace_ordered(obs, pol, ev, sight_xy, emitter: Emitter | None = None,
+                              threads: int = 1):
+    s = np.ascontiguousarray(sight_xy, dtype=np.float64).reshape(-1, 2)
+    n = s.shape[0]
+    out = (Outcome * n)()
+    status = np.zeros(n, dtype=np.int32)
+    em = C.byref(emitter) if emitter is not None else None
+    rc = lib().orc_observation_trace_ordered(C.byref(obs), C.byref(pol), C.byref(ev), em, _dp(s),
+                                             n, threads, out,
+                                             status.ctypes.data_as(C.POINTER(C.c_int)))
+    if rc:
+        raise ValueError("thread count must lie in [1, 256]")
+    return out, status
+
+
+def observable_at(st, emitter, inner, outer, state, omega=1.0):
+    y = np.ascontiguousarray(state, dtype=np.float64)
+    out = Outcome()
+    rc = lib().orc_observable_at(C.byref(st), C.byref(emitter), inner, outer, _dp(y), omega,
+                                 C.byref(out))
+    return rc, out
+
+
+def rhs(st, state):
+    y = np.ascontiguousarray(state, dtype=np.float64)
+    out = np.zeros(8)
+    rc = lib().orc_rhs(C.byref(st), _dp(y), _dp(out))
+    if rc:
+        raise ValueError(f"geometry error {rc}")
+    return out
+
+
+def invariants(st, state):
+    y = np.ascontiguousarray(state, dtype=np.float64)
+    out = np.zeros(4)
+    rc = lib().orc_invariants(C.byref(st), _dp(y), _dp(out))
+    if rc:
+        raise ValueError(f"geometry error {rc}")
+    return out
+
+
+def metric(st, xyz):
+    p = np.ascontiguousarray(xyz, dtype=np.float64)
+    cov = np.zeros(16)
+    inv = np.zeros(16)
+    rc = lib().orc_metric(C.byref(st), _dp(p), _dp(cov), _dp(inv))
+    if rc:
+        raise ValueError(f"geometry error {rc}")
+    return cov.reshape(4, 4), inv.reshape(4, 4)
+
+
+def geometry(st, xyz):
+    p = np.ascontiguousarray(xyz, dtype=np.float64)
+    r = np.zeros(1)
+    f = np.zeros(1)
+    rg = np.zeros(3)
+    fg = np.zeros(3)
+    nc = np.zeros(4)
+    nvg = np.zeros(12)
+    rc = lib().orc_geometry(C.byref(st), _dp(p), _dp(r), _dp(f), _dp(rg), _dp(fg), _dp(nc),
+                            _dp(nvg))
+    if rc:
+        raise ValueError(f"geometry error {rc}")
+    return dict(radius=r[0], f=f[0], radius_gradient=rg, f_gradient=fg, null_covector=nc,
+                null_vector_gradient=nvg.reshape(3, 4))
+
+
+def isco_prograde(spin: float, charge: float) -> float:
+    return lib().orc_isco_prograde(spin, charge)
+
+
+def extremality(mass, spin, charge) -> int:
+    return lib().orc_compare_extremality(mass, spin, charge)
+
+
+def outcomes_to_arrays(outs) -> dict:
+    """Structure-of-arrays view of a ctypes Outcome array (copies)."""
+    n = len(outs)
+    buf = np.frombuffer(outs, dtype=np.uint8).reshape(n, C.sizeof(Outcome))
+    res = {}
+    for name, ctype in Outcome._fields_:
+        off = getattr(Outcome, name).offset
+        size = C.sizeof(ctype)
+        raw = np.ascontiguousarray(buf[:, off:off + size])
+        if ctype in (C.c_double,):
+            res[name] = raw.view(np.float64).reshape(n)
+        elif ctype in (C.c_int, C.c_int32):
+            res[name] = raw.view(np.int32).reshape(n)
+        elif ctype is C.c_uint32:
+            res[name] = raw.view(np.uint32).reshape(n)
+        elif ctype is C.c_uint64:
+            res[name] = raw.view(np.uint64).reshape(n)
+        else:  # fixed arrays of double
+            res[name] = raw.view(np.float64).reshape(n, -1)
+    return res
