@@ -1,0 +1,193 @@
+#!/usr/bin/env python3
+"""Throughput of the Kerr-Newman null-geodesic trace on one MI355X.
+
+Workload: one 1920x1080 frame of Kerr a = 0.94 (Q = 0) seen from r = 100 M at
+80 deg inclination, FOV 15 deg; adaptive f32 Dormand-Prince 5(4) with
+rtol 1e-6 / atol 1e-8; a thin prograde disk [r_ISCO, 30 M] and escape to
+r = 1000 M looked up in a seeded 2048x1024 equirectangular RGB map. A step is
+one full frame (2,073,600 rays) traced by the persistent kernel; the plain
+one-ray-per-lane kernel is timed on the same frame for the ratio.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--accuracy-rays", type=int, default=2048)
+    return ap.parse_args()
+
+
+def angle(a, b):
+    cr = np.linalg.norm(np.cross(a, b), axis=-1)
+    return np.arctan2(cr, np.sum(a * b, axis=-1))
+
+
+def cpu_baseline(scene, gpu_host, seconds, accuracy_rays, seed=20261203):
+    """The oracle (CPU f64 DP5 restatement of the reference trace) on the host
+    cores: a timed sample at the workload's tolerance, and a tight-tolerance
+    sample whose observables measure the f32 error of the GPU frame."""
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from oracle import oracle as O
+
+    cores = max(1, min(16, os.cpu_count() or 1))
+    obs = O.observation(1.0, scene.spin, scene.charge, float(scene.chart),
+                        [0.0, *scene.observer_xyz()], [0, 0, 0, 0], [0, 0, 1], 1.0,
+                        scene.width, scene.height, np.radians(scene.fov_deg))
+    ev = O.events(scene.escape_radius, scene.disk_radii(), horizon_offset=scene.horizon_offset)
+    em = O.Emitter()
+    rng = np.random.default_rng(seed)
+    # timed leg: the workload's own tolerance
+    pol = O.config_policy(scene.rtol, scene.atol)
+    done, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        idx = rng.integers(0, scene.rays, 1024)
+        sxy = O.sight_planes(obs, idx % scene.width, idx // scene.width)
+        O.observation_trace_ordered(obs, pol, ev, sxy, em, threads=cores)
+        done += len(idx)
+    elapsed = time.perf_counter() - t0
+    base = {"value": done / elapsed, "unit": "rays/s", "cores": cores, "kind": "port",
+            "sample": f"{done} seeded pixels of the 1920x1080 frame, f64 DP5 rtol {scene.rtol:g} "
+                      f"atol {scene.atol:g}, {elapsed:.1f} s on {cores} threads"}
+    # accuracy leg: f64 reference at rtol 1e-10, and its 1/16 refinement for
+    # the near-critical flag (termination changes between the two)
+    idx = rng.choice(scene.rays, accuracy_rays, replace=False)
+    sxy = O.sight_planes(obs, idx % scene.width, idx // scene.width)
+    ref, _ = O.observation_trace_ordered(obs, O.config_policy(1e-10, 1e-12), ev, sxy, em,
+                                         threads=cores)
+    fine, _ = O.observation_trace_ordered(obs, O.config_policy(1e-10 / 16, 1e-12 / 16), ev, sxy,
+                                          em, threads=cores)
+    A = O.outcomes_to_arrays(ref)
+    F = O.outcomes_to_arrays(fine)
+    flagged = (A["termination"] != F["termination"]) | (A["candidates"] != F["candidates"])
+    gt = gpu_host["termination"][idx]
+    gc = gpu_host["candidates"][idx]
+    ok = ~flagged
+    agree = float(np.mean((gt == A["termination"])[ok])) if ok.any() else float("nan")
+    agree_ev = float(np.mean(((gt == A["termination"]) & (gc == A["candidates"]))[ok])) if ok.any() else float("nan")
+    err = {"rays": int(accuracy_rays), "near_critical_flagged": int(flagged.sum()),
+           "termination_agreement": agree, "event_id_agreement": agree_ev}
+    esc = ok & (gt == 2) & (A["termination"] == 2)
+    if esc.any():
+        a = angle(gpu_host["observable"][idx][esc], A["escape_direction"][esc])
+        tt = np.abs(gpu_host["travel_time"][idx][esc] - A["travel_time"][esc])
+        err.update(escape_angle_p50=float(np.median(a)), escape_angle_p99=float(np.quantile(a, 0.99)),
+                   escape_angle_max=float(a.max()), travel_time_abs_p99=float(np.quantile(tt, 0.99)),
+                   travel_time_rel_max=float(np.max(tt / np.abs(A["travel_time"][esc]))))
+    srf = ok & (gt == 7) & (A["termination"] == 7)
+    if srf.any():
+        g = gpu_host["observable"][idx][srf]
+        rel = np.abs(g[:, 2] / A["frequency_ratio"][srf] - 1.0)
+        err.update(g_rel_p99=float(np.quantile(rel, 0.99)), g_rel_max=float(rel.max()),
+                   source_r_abs_max=float(np.max(np.abs(g[:, 0] - A["source_radius"][srf]))))
+    err["null_drift_p99"] = float(np.quantile(gpu_host["drift"][idx][:, 0], 0.99))
+    return base, err
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(rays are independent; see DESIGN.md for how the trace would shard)",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import gravlume_amd as g
+    from gravlume_amd import scene as S
+
+    scene = S.single_frame_1080p()
+    pipe = g.TracePipeline([scene])
+    pipe.enable_timing(True)
+    rhs_flops, attempt_flops = g.flop_constants()
+    target = pipe.create_target(0)
+    plain_target = pipe.create_target(0)
+    for _ in range(args.warmup):
+        pipe.encode_persistent([target])
+        pipe.encode_batch(plain_target)
+    torch.cuda.synchronize()
+
+    def timed(fn):
+        main_ms, fin_ms = [], []
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            fn()
+            m, f = pipe.kernel_ms()
+            main_ms.append(m)
+            fin_ms.append(f)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, float(np.mean(main_ms)), float(np.mean(fin_ms))
+
+    t_pers, k_pers, f_pers = timed(lambda: pipe.encode_persistent([target]))
+    stats = pipe.queue_stats()
+    t_plain, k_plain, f_plain = timed(lambda: pipe.encode_batch(plain_target))
+    host = target.to_host()
+    plain_host = plain_target.to_host()
+    rays = scene.rays
+    value = rays * args.steps / t_pers
+    plain_value = rays * args.steps / t_plain
+    attempts = int(host["steps"].sum() + host["rejected"].sum())
+    flops = attempts * attempt_flops
+    achieved = flops / (k_pers * 1e-3) / 1e12
+    peak = 157.3
+    agree_plain = float(np.mean(host["termination"] == plain_host["termination"]))
+
+    line = {
+        "metric": "rays/s @1080p Kerr a=0.94, plain vs persistent; VJP cost ratio; err vs f64",
+        "value": value,
+        "unit": "rays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * t_pers / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "single frame 1920x1080, Kerr a=0.94 Q=0, observer r=100M incl 80deg, "
+                               "FOV 15deg, f32 DP5(4) rtol 1e-6 atol 1e-8, disk [r_ISCO, 30M], "
+                               "escape r=1000M into a 2048x1024 seeded RGB map",
+                   "rays_per_step": rays, "kernel": "k_persistent"},
+        "plain_rays_per_s": plain_value,
+        "persistent_over_plain": value / plain_value,
+        "kernel_ms": {"persistent": k_pers, "persistent_f64_finisher": f_pers, "plain": k_plain,
+                      "plain_f64_finisher": f_plain},
+        "plain_persistent_termination_agreement": agree_plain,
+        "steps_per_ray": {"accepted_mean": float(host["steps"].mean()),
+                          "rejected_mean": float(host["rejected"].mean()),
+                          "accepted_max": int(host["steps"].max())},
+        "lane_utilization": stats["attempted_steps"] / max(1, stats["lane_slots"]),
+        "terminations": {int(k): int(v) for k, v in zip(*np.unique(host["termination"], return_counts=True))},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                     "frac": achieved / peak, "frac_of_nonpacked_fp32": achieved / 78.6,
+                     "traffic": None,
+                     "flops_per_launch": flops, "flops_per_attempt": attempt_flops,
+                     "flops_per_rhs": rhs_flops, "attempts_per_launch": attempts},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        base, err = cpu_baseline(scene, host, args.cpu_seconds, args.accuracy_rays)
+        line["cpu_baseline"] = base
+        line["error_vs_f64"] = err
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,19 @@
+"""MI355X (gfx950) Kerr-Newman null-geodesic trace.
+
+The trace runs in hand-written HIP kernels (gravlume_amd/csrc) behind the C
+ABI of include/gravlume.h; this package is the host-side mirror of the
+reference's trace seams. Importing it loads libgravlume_hip.so and fails if
+the library has not been built: there is no CPU fallback.
+"""
+from . import _capi
+from ._capi import GravlumeError
+
+_capi.load()
+
+from .scene import Scene, escape_map, isco_prograde, outer_horizon  # noqa: E402
+from .trace import (FrameGraph, GeodesicBatch, GpuTraceInputError, TileRegion,  # noqa: E402
+                    TracePipeline, TraceTarget, flop_constants)
+
+__all__ = ["FrameGraph", "GeodesicBatch", "GpuTraceInputError", "GravlumeError", "Scene",
+           "TileRegion", "TracePipeline", "TraceTarget", "escape_map", "flop_constants",
+           "isco_prograde", "outer_horizon"]

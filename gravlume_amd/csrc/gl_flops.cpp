@@ -1,0 +1,161 @@
+// Flop constants of the f32 trace, counted from the kernel source itself:
+// the templated geometry, right-hand side and DP5 step are instantiated on a
+// host operation-counting scalar. Every add, subtract, multiply, divide and
+// square root counts one operation (a fused multiply-add is two), and each
+// transcendental call (atan2, exp2, log2, pow) counts one.
+#include <cmath>
+#include <cstdint>
+
+#include "../../include/gravlume.h"
+#include "ks_math.h"
+
+namespace gl {
+
+struct CF {
+  double v;
+  static uint64_t n;
+  CF() : v(0.0) {}
+  CF(double x) : v(x) {}  // NOLINT: implicit from literals like T(0.5)
+  explicit operator double() const { return v; }
+  explicit operator float() const { return float(v); }
+  CF operator-() const { return CF(-v); }
+  CF& operator+=(CF o) {
+    ++n;
+    v += o.v;
+    return *this;
+  }
+  CF& operator-=(CF o) {
+    ++n;
+    v -= o.v;
+    return *this;
+  }
+  CF& operator*=(CF o) {
+    ++n;
+    v *= o.v;
+    return *this;
+  }
+  CF& operator/=(CF o) {
+    ++n;
+    v /= o.v;
+    return *this;
+  }
+};
+uint64_t CF::n = 0;
+inline CF operator+(CF a, CF b) { return a += b; }
+inline CF operator-(CF a, CF b) { return a -= b; }
+inline CF operator*(CF a, CF b) { return a *= b; }
+inline CF operator/(CF a, CF b) { return a /= b; }
+inline bool operator<(CF a, CF b) { return a.v < b.v; }
+inline bool operator>(CF a, CF b) { return a.v > b.v; }
+inline bool operator<=(CF a, CF b) { return a.v <= b.v; }
+inline bool operator>=(CF a, CF b) { return a.v >= b.v; }
+inline bool operator==(CF a, CF b) { return a.v == b.v; }
+inline bool operator!=(CF a, CF b) { return a.v != b.v; }
+
+template <>
+inline CF gl_sqrt<CF>(CF x) {
+  ++CF::n;
+  return CF(std::sqrt(x.v));
+}
+template <>
+inline CF gl_rcp<CF>(CF x) {
+  ++CF::n;
+  return CF(1.0 / x.v);
+}
+template <>
+inline CF gl_rsqrt<CF>(CF x) {
+  ++CF::n;
+  return CF(1.0 / std::sqrt(x.v));
+}
+
+}  // namespace gl
+
+#include "trace_device.h"
+
+namespace gl {
+
+template <>
+inline CF gl_atan2<CF>(CF y, CF x) {
+  ++CF::n;
+  return CF(std::atan2(y.v, x.v));
+}
+template <>
+inline CF pow_m02<CF>(CF v) {
+  CF::n += 3;  // multiply, exp2, log2
+  return CF(std::pow(v.v, -0.2));
+}
+template <>
+inline CF gl_log2<CF>(CF v) {
+  ++CF::n;
+  return CF(std::log2(v.v));
+}
+template <>
+inline CF gl_exp2<CF>(CF v) {
+  ++CF::n;
+  return CF(std::exp2(v.v));
+}
+template <>
+inline CF gl_floor<CF>(CF v) {
+  return CF(std::floor(v.v));
+}
+
+namespace {
+
+// One geometry + right-hand side evaluation at a generic off-axis point.
+uint64_t count_rhs() {
+  CF::n = 0;
+  Geom<CF> g = geometry<CF>(CF(7.3), CF(-4.1), CF(2.2), CF(1.0), CF(0.94), CF(0.94 * 0.94),
+                            CF(0.0), CF(-1.0));
+  Deriv<CF> d = rhs_from_geometry<CF>(g, CF(7.3), CF(-4.1), CF(2.2), CF(-0.6), CF(0.4),
+                                      CF(-0.3), CF(1.0), CF(0.94), CF(-1.0));
+  (void)d;
+  return CF::n;
+}
+
+// One accepted, non-terminal DP5 attempt (six RHS evaluations included)
+// with the drift, winding, arming and controller bookkeeping, no
+// re-projection.
+uint64_t count_attempt() {
+  SceneT<CF> sc;
+  sc.mass = 1.0;
+  sc.spin = 0.94;
+  sc.spin2 = 0.94 * 0.94;
+  sc.charge2 = 0.0;
+  sc.chart = -1.0;
+  sc.r_capture = 1.3411744 * (1 + 1e-6);
+  sc.r_horizon = 1.3411744;
+  sc.r_escape = 1000.0;
+  sc.disk_in = 2.0235931;
+  sc.disk_out = 30.0;
+  sc.band = 1.0 / 4096.0;
+  sc.rtol = 1e-6;
+  sc.atol = 1e-8;
+  sc.h_min = 1e-6;
+  sc.h_max = 64.0;
+  sc.h_max_per_r = 1.0;
+  sc.tie_tol = 1.0 / 131072.0;
+  sc.drift_limit = 0.05;
+  sc.guard = std::ldexp(1.0, -40);
+  sc.max_accepted = 100000;
+  sc.max_rejects = 64;
+  sc.options = 0;
+  sc.reproject_interval = 0;
+  RayT<CF> R;
+  init_ray<CF>(sc, R, CF(40.0), CF(10.0), CF(30.0), CF(0.0), CF(0.0), CF(0.0), CF(-0.7),
+               CF(-0.1), CF(-0.6), CF(1.0), -1.0f, 2u);
+  R.h = CF(-1e-3);
+  R.t = CF(-5.0);
+  CF::n = 0;
+  step<CF>(sc, R, CF(1.0));
+  return CF::n;
+}
+
+}  // namespace
+}  // namespace gl
+
+extern "C" gl_status gl_flop_constants(double* rhs_flops, double* attempt_flops) {
+  if (!rhs_flops || !attempt_flops) return GL_ERR_INVALID_ARGUMENT;
+  *rhs_flops = double(gl::count_rhs());
+  *attempt_flops = double(gl::count_attempt());
+  return GL_OK;
+}

@@ -1,0 +1,84 @@
+// Device-resident scene record and launch bookkeeping shared by the host
+// packer (gl_scene.cpp) and the kernels (gl_trace.hip).
+#pragma once
+
+#include <stdint.h>
+
+namespace gl {
+
+// One scene as the kernels read it: the f32 image of the validated f64 scene
+// plus the few f64 values the f64 islands need. 256 bytes, read once per
+// ray into registers (the role of TraceUniforms, trace/input.rs:19-35).
+struct alignas(16) SceneDev {
+  // spacetime
+  float mass, spin, spin2, charge2;
+  float chart;          // +1 ingoing, -1 outgoing
+  float r_capture;      // r_+ (1 + offset); < 0 without a horizon
+  float r_horizon;      // r_+; < 0 without a horizon
+  float r_escape;
+  // surface
+  float disk_in, disk_out;  // disk_in < 0 disables
+  float temperature_inner;  // T at disk_in
+  float arming_band;
+  // policy
+  float rtol, atol;
+  float h_init, h_min, h_max, h_max_per_r;
+  float tie_tol, drift_limit;
+  float guard;  // singularity guard on r^4 + a^2 z^2
+  uint32_t max_accepted, max_rejects;
+  uint32_t options;
+  uint32_t reproject_interval;
+  // observer and view
+  float obs_x, obs_y, obs_z;
+  float u_t;                        // observer four-velocity, time component
+  float right[4], up[4], arrival[4];  // tetrad legs (t, x, y, z)
+  float tan_half_fov, aspect;
+  float sub_x, sub_y;
+  float omega;
+  uint32_t width, height;
+  uint32_t polar_side;  // 0 negative, 1 equatorial, 2 positive
+  uint32_t ray_offset;  // first work item of this scene in a persistent frame
+  uint32_t tiles_x;     // ceil(width / 8)
+  // f64 copies for the f64 islands
+  double spin_d, charge2_d, r_horizon_d;
+  double r_capture_d, disk_in_d, disk_out_d;
+};
+
+static_assert(sizeof(SceneDev) == 256, "SceneDev must stay 256 bytes");
+
+// Work-queue and statistics block of one persistent launch.
+struct alignas(64) QueueDev {
+  unsigned long long next;      // next unclaimed work item
+  unsigned long long total;     // items in the frame
+  unsigned long long refills;
+  unsigned long long attempts;  // DP5 attempts by active lanes
+  unsigned long long slots;     // lane-attempt slots offered
+  unsigned long long pad[3];
+};
+
+// Output planes as the kernels see them (see gl_ray_outputs in gravlume.h).
+struct OutPlanes {
+  uint32_t* status;
+  uint32_t* steps;
+  uint32_t* rejected;
+  float* observable;
+  float* travel_time;
+  float* affine;
+  float* drift;
+  uint32_t* branch;
+  float* radiance;
+  unsigned long long count;
+};
+
+// Per-scene slice of a persistent frame.
+struct FrameScene {
+  uint32_t scene;       // index into the scene table
+  uint32_t first_item;  // prefix sum of rays
+  uint32_t rays;
+  uint32_t pad;
+  OutPlanes out;
+};
+
+constexpr int kMaxScenes = 4096;
+
+}  // namespace gl

@@ -1,0 +1,267 @@
+"""Host mirror of the reference's GPU trace seam on top of the C ABI.
+
+Names follow crates/gravlume-render/src/trace.rs: a ``TracePipeline`` owns the
+packed scenes (TracePipeline::new, trace.rs:320-327), ``create_target``
+allocates the per-ray output planes (trace.rs:565-663) and ``encode_batch``
+traces a ``TileRegion`` of 8x8 tiles (trace.rs:701-857). ``encode_persistent``
+traces whole scenes in one persistent launch; ``GeodesicBatch`` mirrors
+gravlume-reference's GeodesicBatch::trace_ordered (batch.rs:19-43) on the GPU.
+PyTorch only provides device memory and the stream.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _capi
+from .scene import Scene, escape_map
+
+
+class GpuTraceInputError(ValueError):
+    """The scene failed validation or the f32 packing (input.rs:342-381)."""
+
+
+@dataclass
+class TileRegion:
+    """A rectangle of 8x8 tiles of one scene (trace.rs:799-857)."""
+    scene: int
+    tile_x: int
+    tile_y: int
+    tiles_x: int
+    tiles_y: int
+
+    @staticmethod
+    def full(scene_index: int, width: int, height: int) -> "TileRegion":
+        return TileRegion(scene_index, 0, 0, (width + 7) // 8, (height + 7) // 8)
+
+
+class TraceTarget:
+    """Caller-owned structure-of-arrays outputs for n rays on the device."""
+
+    def __init__(self, rays: int, device: str | torch.device = "cuda"):
+        dev = torch.device(device)
+        self.rays = rays
+        self.status = torch.zeros(rays, dtype=torch.int32, device=dev)
+        self.steps = torch.zeros(rays, dtype=torch.int32, device=dev)
+        self.rejected = torch.zeros(rays, dtype=torch.int32, device=dev)
+        self.observable = torch.zeros(3 * rays, dtype=torch.float32, device=dev)
+        self.travel_time = torch.zeros(rays, dtype=torch.float32, device=dev)
+        self.affine = torch.zeros(rays, dtype=torch.float32, device=dev)
+        self.drift = torch.zeros(3 * rays, dtype=torch.float32, device=dev)
+        self.branch = torch.zeros(rays, dtype=torch.int32, device=dev)
+        self.radiance = torch.zeros(3 * rays, dtype=torch.float32, device=dev)
+
+    def c_struct(self) -> _capi.RayOutputsC:
+        o = _capi.RayOutputsC()
+        for name in ("status", "steps", "rejected", "observable", "travel_time", "affine",
+                     "drift", "branch", "radiance"):
+            setattr(o, name, getattr(self, name).data_ptr())
+        o.count = self.rays
+        return o
+
+    def to_host(self) -> dict:
+        """Decoded numpy copies of every plane."""
+        n = self.rays
+        st = self.status.cpu().numpy().view(np.uint32)
+        br = self.branch.cpu().numpy().view(np.uint32)
+        out = {
+            "termination": (st & 15).astype(np.int32),
+            "candidates": ((st >> 4) & 15).astype(np.int32),
+            "flags": ((st >> 8) & 255).astype(np.int32),
+            "steps": self.steps.cpu().numpy().view(np.uint32).astype(np.int64),
+            "rejected": self.rejected.cpu().numpy().view(np.uint32).astype(np.int64),
+            "observable": self.observable.cpu().numpy().reshape(3, n).T.copy(),
+            "travel_time": self.travel_time.cpu().numpy(),
+            "affine": self.affine.cpu().numpy(),
+            "drift": self.drift.cpu().numpy().reshape(3, n).T.copy(),
+            "radial_turnings": (br & 255).astype(np.int32),
+            "equatorial_crossings": ((br >> 8) & 255).astype(np.int32),
+            "azimuth_winding": ((br >> 16) & 255).astype(np.uint8).view(np.int8).astype(np.int32),
+            "polar_side": ((br >> 24) & 3).astype(np.int32),
+            "radiance": self.radiance.cpu().numpy().reshape(3, n).T.copy(),
+        }
+        return out
+
+
+def _stream_handle(stream) -> C.c_void_p:
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    return C.c_void_p(stream.cuda_stream)
+
+
+class TracePipeline:
+    """Validated, packed scenes resident on the GPU (TracePipeline::new)."""
+
+    def __init__(self, scenes: list[Scene] | Scene, escape_map_rgb: np.ndarray | None = None,
+                 map_seed: int = 20261203):
+        self._lib = _capi.load()
+        if isinstance(scenes, Scene):
+            scenes = [scenes]
+        self.scenes = list(scenes)
+        descs = (_capi.SceneDesc * len(self.scenes))(*[s.to_desc() for s in self.scenes])
+        if escape_map_rgb is None and any(s.escape_map for s in self.scenes):
+            escape_map_rgb = escape_map(map_seed)
+        self.escape_map_rgb = escape_map_rgb
+        mp, mw, mh = None, 0, 0
+        if escape_map_rgb is not None:
+            arr = np.ascontiguousarray(escape_map_rgb, dtype=np.float32)
+            mp, mh, mw = arr.ctypes.data_as(C.c_void_p), arr.shape[1], arr.shape[2]
+        handle = C.c_void_p()
+        torch.cuda.init()
+        st = self._lib.gl_trace_plan_create(descs, len(self.scenes), mp, mw, mh, C.byref(handle))
+        if st != _capi.GL_OK:
+            msg = self._lib.gl_last_error().decode()
+            if st in (1, 2, 3, 5):
+                raise GpuTraceInputError(msg)
+            raise _capi.GravlumeError(st, msg)
+        self._handle = handle
+
+    def __del__(self):
+        h = getattr(self, "_handle", None)
+        if h is not None and h.value:
+            self._lib.gl_trace_plan_destroy(h)
+            self._handle = None
+
+    @property
+    def handle(self) -> C.c_void_p:
+        return self._handle
+
+    def scene_rays(self, scene: int) -> int:
+        n = C.c_size_t()
+        _capi.check(self._lib.gl_trace_plan_scene_rays(self._handle, scene, C.byref(n)))
+        return n.value
+
+    def create_target(self, scene: int = 0, device="cuda") -> TraceTarget:
+        return TraceTarget(self.scene_rays(scene), device)
+
+    def encode_batch(self, target: TraceTarget, region: TileRegion | None = None,
+                     stream=None) -> None:
+        """Plain one-ray-per-lane trace of a tile region (encode_batch)."""
+        if region is None:
+            s = self.scenes[0]
+            region = TileRegion.full(0, s.width, s.height)
+        r = _capi.TileRegionC(region.scene, region.tile_x, region.tile_y, region.tiles_x,
+                              region.tiles_y)
+        out = target.c_struct()
+        _capi.check(self._lib.gl_trace_encode(self._handle, C.byref(r), C.byref(out),
+                                              _stream_handle(stream)))
+
+    def encode_persistent(self, targets: list[TraceTarget] | TraceTarget,
+                          scenes: list[int] | None = None, stream=None) -> None:
+        """All rays of `scenes` (default: every scene) in one persistent launch."""
+        if isinstance(targets, TraceTarget):
+            targets = [targets]
+        outs = (_capi.RayOutputsC * len(targets))(*[t.c_struct() for t in targets])
+        if scenes is None:
+            ids, n = None, len(self.scenes)
+        else:
+            arr = (C.c_uint32 * len(scenes))(*scenes)
+            ids, n = C.cast(arr, C.c_void_p), len(scenes)
+        if len(targets) != n:
+            raise ValueError("one target per traced scene is required")
+        _capi.check(self._lib.gl_trace_encode_persistent(self._handle, ids, n, outs,
+                                                         _stream_handle(stream)))
+
+    def set_persistent_config(self, block_size=256, blocks_per_cu=2, refill_interval=8,
+                              refill_threshold=16) -> None:
+        c = _capi.PersistentConfig(block_size, blocks_per_cu, refill_interval, refill_threshold)
+        _capi.check(self._lib.gl_trace_plan_set_persistent_config(self._handle, C.byref(c)))
+
+    def persistent_config(self) -> dict:
+        c = _capi.PersistentConfig()
+        _capi.check(self._lib.gl_trace_plan_get_persistent_config(self._handle, C.byref(c)))
+        return {k: getattr(c, k) for k, _ in c._fields_}
+
+    def queue_stats(self, stream=None) -> dict:
+        s = _capi.QueueStats()
+        _capi.check(self._lib.gl_trace_plan_queue_stats(self._handle, C.byref(s),
+                                                        _stream_handle(stream)))
+        return {k: getattr(s, k) for k, _ in s._fields_}
+
+    def enable_timing(self, enable: bool = True) -> None:
+        """HIP events around the main kernel and the f64 finisher of each launch."""
+        _capi.check(self._lib.gl_trace_plan_enable_timing(self._handle, int(enable)))
+
+    def kernel_ms(self) -> tuple[float, float]:
+        """(main kernel ms, f64 finisher ms) of the last launch."""
+        a, b = C.c_double(), C.c_double()
+        _capi.check(self._lib.gl_trace_plan_kernel_ms(self._handle, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def frame_graph(self, targets, scenes=None, stream=None) -> "FrameGraph":
+        return FrameGraph(self, targets, scenes, stream)
+
+
+class FrameGraph:
+    """The persistent frame (queue reset + launch) captured once as a hipGraph."""
+
+    def __init__(self, pipeline: TracePipeline, targets, scenes=None, stream=None):
+        self._lib = pipeline._lib
+        self.pipeline = pipeline
+        if isinstance(targets, TraceTarget):
+            targets = [targets]
+        self.targets = targets
+        outs = (_capi.RayOutputsC * len(targets))(*[t.c_struct() for t in targets])
+        if scenes is None:
+            ids, n = None, len(pipeline.scenes)
+        else:
+            self._ids = (C.c_uint32 * len(scenes))(*scenes)
+            ids, n = C.cast(self._ids, C.c_void_p), len(scenes)
+        self.stream = stream if stream is not None else torch.cuda.Stream()
+        h = C.c_void_p()
+        with torch.cuda.stream(self.stream):
+            _capi.check(self._lib.gl_frame_graph_create(pipeline.handle, ids, n, outs,
+                                                        _stream_handle(self.stream), C.byref(h)))
+        self._handle = h
+
+    def launch(self) -> None:
+        _capi.check(self._lib.gl_frame_graph_launch(self._handle, _stream_handle(self.stream)))
+
+    def __del__(self):
+        h = getattr(self, "_handle", None)
+        if h is not None and h.value:
+            self._lib.gl_frame_graph_destroy(h)
+            self._handle = None
+
+
+class GeodesicBatch:
+    """Ordered batch of geodesic traces on the GPU (GeodesicBatch::trace_ordered).
+
+    ``trace_ordered(states, directions)`` takes n initial states
+    (t, x, y, z, p_t, p_x, p_y, p_z) in f64 and affine directions (+1 forward,
+    -1 backward); the spacetime, policy and events are those of scene
+    `scene` of the pipeline. Outcome i belongs to request i.
+    """
+
+    def __init__(self, pipeline: TracePipeline, scene: int = 0):
+        self.pipeline = pipeline
+        self.scene = scene
+
+    def trace_ordered(self, states: np.ndarray, directions: np.ndarray, stream=None) -> dict:
+        states = np.ascontiguousarray(states, dtype=np.float64).reshape(-1, 8)
+        n = states.shape[0]
+        dirs = np.ascontiguousarray(directions, dtype=np.float32).reshape(-1)
+        if dirs.shape[0] != n:
+            raise ValueError("one direction per state is required")
+        d_states = torch.from_numpy(states).cuda()
+        d_dirs = torch.from_numpy(dirs).cuda()
+        target = TraceTarget(n)
+        out = target.c_struct()
+        lib = self.pipeline._lib
+        _capi.check(lib.gl_geodesic_batch_trace(self.pipeline.handle, self.scene,
+                                                C.c_void_p(d_states.data_ptr()),
+                                                C.c_void_p(d_dirs.data_ptr()), n, C.byref(out),
+                                                _stream_handle(stream)))
+        torch.cuda.synchronize()
+        return target.to_host()
+
+
+def flop_constants() -> tuple[float, float]:
+    """Counted flops of one RHS evaluation and of one accepted DP5 attempt."""
+    lib = _capi.load()
+    a, b = C.c_double(), C.c_double()
+    _capi.check(lib.gl_flop_constants(C.byref(a), C.byref(b)))
+    return a.value, b.value

@@ -1,0 +1,238 @@
+/*
+ * gravlume.h - C ABI of the MI355X (gfx950) Kerr-Newman null-geodesic trace.
+ *
+ * Drop-in boundary for the reference's two trace seams:
+ *   - the GPU trace dispatch of gravlume-render
+ *       TracePipeline::new      crates/gravlume-render/src/trace.rs:320-327
+ *       TracePipeline::create_target                     trace.rs:565-663
+ *       TracePipeline::encode_batch (+ TileRegion)       trace.rs:701-857
+ *   - the CPU batch API of gravlume-reference
+ *       GeodesicBatch::new / trace_ordered crates/gravlume-reference/src/batch.rs:19-43
+ * Plain C: no torch or HIP types in the signatures. Device buffers are
+ * caller-owned device pointers; `stream` is a hipStream_t passed as void*
+ * (NULL = the default stream). Every entry point returns a gl_status; the
+ * text of the last error on the calling thread is gl_last_error().
+ */
+#ifndef GRAVLUME_H
+#define GRAVLUME_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum gl_status {
+  GL_OK = 0,
+  GL_ERR_INVALID_ARGUMENT = 1,    /* null pointer, size or enum out of range */
+  GL_ERR_DOMAIN = 2,              /* scene fails a physical validity check   */
+  GL_ERR_NOT_REPRESENTABLE = 3,   /* a value does not survive the f32 packing */
+  GL_ERR_DEVICE = 4,              /* HIP runtime failure                     */
+  GL_ERR_CAPACITY = 5             /* request exceeds a documented limit      */
+} gl_status;
+
+/* Termination classes (reference GPU protocol trace_protocol.wgsl:85-91 and
+ * oracle Termination, crates/gravlume-reference/src/outcome.rs:149-158). */
+enum {
+  GL_TERM_ACTIVE = 0,
+  GL_TERM_HORIZON = 1,
+  GL_TERM_ESCAPE = 2,
+  GL_TERM_SINGULARITY = 3,
+  GL_TERM_STEP_EXHAUSTION = 4,
+  GL_TERM_NUMERICAL_FAILURE = 5,
+  GL_TERM_UNCERTAIN = 6,
+  GL_TERM_SURFACE = 7,
+  GL_TERM_REJECT_EXHAUSTION = 8,
+  GL_TERM_CERTIFIED_CAPTURE = 9 /* horizon proven by the interval certificate */
+};
+
+/* Event-candidate bits (trace_protocol.wgsl:95-98, geodesic.rs:616-634). */
+enum {
+  GL_EVENT_SINGULARITY = 1u,
+  GL_EVENT_HORIZON = 2u,
+  GL_EVENT_SURFACE = 4u,
+  GL_EVENT_ESCAPE = 8u
+};
+
+/* Numerical flags (trace_protocol.wgsl:92-94). */
+enum {
+  GL_FLAG_NON_FINITE = 1u,
+  GL_FLAG_INVALID_RADICAND = 2u,
+  GL_FLAG_INVALID_DENOMINATOR = 4u,
+  GL_FLAG_DRIFT_BUDGET = 8u,     /* invariant drift above the scene limit    */
+  GL_FLAG_TURNING_ORDER = 16u,   /* turning bracket overlaps the terminal    */
+  GL_FLAG_TIED_EVENTS = 32u      /* several event candidates within the tie  */
+};
+
+/* Option bits of gl_scene_desc.options. */
+enum {
+  GL_OPT_CAPTURE_CERTIFICATE = 1u, /* interval radial-capture certificate      */
+  GL_OPT_F64_HORIZON_STEPS = 2u,   /* f64 DP5 steps inside 1.05 r_+            */
+  GL_OPT_F64_DISK_ROOT = 4u,       /* f64 equatorial root solve + observables  */
+  GL_OPT_ESCAPE_MAP = 8u           /* look escape directions up in the map     */
+};
+
+/* One scene: the content of the reference's validated Observation
+ * (crates/gravlume-domain/src/scene.rs:99-143, view.rs:28-69), its event
+ * configuration (crates/gravlume-reference/src/events.rs:17-105) and the
+ * integrator policy. All lengths in units of M; mass must be exactly 1
+ * (geodesic.rs:28-41). */
+typedef struct gl_scene_desc {
+  double mass, spin, charge;
+  int32_t chart; /* +1 ingoing, -1 outgoing Kerr-Schild chart */
+  uint32_t options;
+  double observer_txyz[4]; /* stationary observer event (Cartesian KS) */
+  double target_txyz[4];   /* image-centre target */
+  double up_hint[3];       /* image-up hint */
+  double frequency;        /* observer frequency, must be 1 (input.rs:57-63) */
+  uint32_t width, height;  /* viewport; pixel (0,0) is top-left */
+  double vertical_fov;     /* radians, in (0, pi) */
+  double subpixel[2];      /* sample offset inside each pixel, in [0,1] */
+  double escape_radius;    /* escape event r = escape_radius */
+  double horizon_offset;   /* capture event r = r_+ (1 + horizon_offset) */
+  int32_t has_surface;     /* equatorial prograde Keplerian emitter */
+  double surface_inner, surface_outer;
+  double temperature_inner; /* T at surface_inner [K]; T ~ r^-3/4 */
+  /* adaptive Dormand-Prince 5(4) policy */
+  double rtol, atol;
+  double initial_step, minimum_step, maximum_step;
+  double maximum_step_per_radius; /* |h| <= this * r (0 disables) */
+  uint32_t max_accepted_steps, max_consecutive_rejects;
+  double event_tie_tolerance; /* M */
+  double arming_band;         /* M */
+  double drift_limit;         /* relative invariant drift budget */
+  uint32_t reproject_interval; /* f64 null re-projection every N accepted steps (0 off) */
+  uint32_t reserved;
+} gl_scene_desc;
+
+/* Rectangle of 8x8 tiles of one scene (TileRegion, trace.rs:799-857). */
+typedef struct gl_tile_region {
+  uint32_t scene;
+  uint32_t tile_x, tile_y;
+  uint32_t tiles_x, tiles_y;
+} gl_tile_region;
+
+/* Structure-of-arrays per-ray outputs, caller-owned device memory, one
+ * element per ray (index = y * width + x for image traces, request order
+ * for geodesic batches). Any pointer may be NULL to skip that plane.
+ *   status      bits 0-3 termination, 4-7 event candidates, 8-15 flags
+ *   steps       accepted DP5 steps; rejected in `rejected`
+ *   observable  escape: unit traversal direction (x,y,z);
+ *               surface: (r, phi_s, g) source anchor and frequency ratio
+ *   travel_time coordinate-time duration along the traversal (>= 0)
+ *   affine      affine parameter at termination (negative for backward)
+ *   drift       (max null residual, max |dL_z|, max |dQ|) relative drifts
+ *   branch      bits 0-7 radial turnings, 8-15 equatorial crossings,
+ *               16-23 azimuth winding (two's complement), 24-25 polar side
+ *   radiance    escape: escape-map RGB; surface: (g T(r) [K], g^4 r^-3, 0) */
+typedef struct gl_ray_outputs {
+  uint32_t* status;
+  uint32_t* steps;
+  uint32_t* rejected;
+  float* observable; /* 3 planes of n floats: [0,n) [n,2n) [2n,3n) */
+  float* travel_time;
+  float* affine;
+  float* drift;      /* 3 planes */
+  uint32_t* branch;
+  float* radiance;   /* 3 planes */
+  size_t count;      /* number of rays the planes hold */
+} gl_ray_outputs;
+
+typedef struct gl_trace_plan gl_trace_plan;
+
+/* Validates and packs n_scenes scenes (TracePipeline::new, input.rs:45-131)
+ * and uploads them with the escape map (width x height texels, 3 float
+ * planes on the host, may be NULL). */
+gl_status gl_trace_plan_create(const gl_scene_desc* scenes, size_t n_scenes,
+                               const float* escape_map_rgb, uint32_t map_width,
+                               uint32_t map_height, gl_trace_plan** out_plan);
+gl_status gl_trace_plan_destroy(gl_trace_plan* plan);
+
+/* Number of rays of a scene (width * height). */
+gl_status gl_trace_plan_scene_rays(const gl_trace_plan* plan, uint32_t scene, size_t* out_rays);
+
+/* Plain trace: one ray per lane over the tiles of `region`, 8x8 tiles per
+ * 64-lane wavefront (encode_batch, trace.rs:701-784). */
+gl_status gl_trace_encode(gl_trace_plan* plan, const gl_tile_region* region,
+                          const gl_ray_outputs* out, void* stream);
+
+/* Persistent trace: every ray of the listed scenes (all scenes if scenes is
+ * NULL) in one launch from a device work queue with wavefront refill.
+ * outputs[i] receives scene scenes[i]. */
+gl_status gl_trace_encode_persistent(gl_trace_plan* plan, const uint32_t* scenes,
+                                     size_t n_scenes, const gl_ray_outputs* outputs,
+                                     void* stream);
+
+/* Launch parameters of the persistent kernel (set by the autotuner). */
+typedef struct gl_persistent_config {
+  uint32_t block_size;       /* lanes per workgroup: 64..1024, multiple of 64 */
+  uint32_t blocks_per_cu;    /* resident workgroups per CU */
+  uint32_t refill_interval;  /* DP5 attempts between refill checks */
+  uint32_t refill_threshold; /* free lanes that trigger a refill */
+} gl_persistent_config;
+gl_status gl_trace_plan_set_persistent_config(gl_trace_plan* plan,
+                                              const gl_persistent_config* config);
+gl_status gl_trace_plan_get_persistent_config(const gl_trace_plan* plan,
+                                              gl_persistent_config* config);
+
+/* Geodesic batch (GeodesicBatch::trace_ordered, batch.rs:31-43) on the GPU:
+ * n requests of scene `scene`'s spacetime, policy and events, each an
+ * initial state (t, x, y, z, p_t, p_x, p_y, p_z) in f64 on the device and an
+ * affine direction (+1 forward, -1 backward). Results in request order. */
+gl_status gl_geodesic_batch_trace(gl_trace_plan* plan, uint32_t scene,
+                                  const double* states8, const float* directions, size_t n,
+                                  const gl_ray_outputs* out, void* stream);
+
+/* Persistent-queue statistics of the last persistent launch. */
+typedef struct gl_queue_stats {
+  uint64_t items;          /* rays handed out */
+  uint64_t refills;        /* wavefront refill rounds */
+  uint64_t attempted_steps;/* DP5 attempts over all lanes */
+  uint64_t lane_slots;     /* lane-attempt slots offered (active or idle) */
+} gl_queue_stats;
+gl_status gl_trace_plan_queue_stats(gl_trace_plan* plan, gl_queue_stats* stats, void* stream);
+
+/* Kernel timing: when enabled, HIP events on the launch stream bracket the
+ * main trace kernel and the f64 near-horizon finisher of every subsequent
+ * encode call (not inside graph capture). gl_trace_plan_kernel_ms
+ * synchronizes on those events and returns the last launch's durations. */
+gl_status gl_trace_plan_enable_timing(gl_trace_plan* plan, int enable);
+gl_status gl_trace_plan_kernel_ms(gl_trace_plan* plan, double* main_ms, double* finish_ms);
+
+/* Frame capture: records the persistent frame (queue reset + launch) into a
+ * hipGraph once; gl_frame_graph_launch replays it. */
+typedef struct gl_frame_graph gl_frame_graph;
+gl_status gl_frame_graph_create(gl_trace_plan* plan, const uint32_t* scenes, size_t n_scenes,
+                                const gl_ray_outputs* outputs, void* stream,
+                                gl_frame_graph** out_graph);
+gl_status gl_frame_graph_launch(gl_frame_graph* graph, void* stream);
+gl_status gl_frame_graph_destroy(gl_frame_graph* graph);
+
+/* Discrete-adjoint vector-Jacobian product of per-pixel frequency ratios g
+ * with respect to (spin a, charge Q, inclination, pixel direction), following
+ * the torch.autograd.Function contract: forward saves state, backward takes
+ * dL/dg per pixel. Scene `scene` must be an image scene with a surface. */
+typedef struct gl_vjp_state gl_vjp_state;
+gl_status gl_vjp_forward(gl_trace_plan* plan, uint32_t scene, double inclination,
+                         uint32_t checkpoint_interval, const gl_ray_outputs* out, void* stream,
+                         gl_vjp_state** out_state);
+/* grad_g: device float[n_rays]; grad_params: host double[3] (a, Q, incl);
+ * grad_direction: device float[2 * n_rays] (d/d sight-plane x, y), may be NULL. */
+gl_status gl_vjp_backward(gl_vjp_state* state, const float* grad_g, double* grad_params,
+                          float* grad_direction, void* stream);
+gl_status gl_vjp_destroy(gl_vjp_state* state);
+
+/* Per-thread text of the last error. */
+const char* gl_last_error(void);
+
+/* Library build identity and the counted per-RHS / per-attempt flop
+ * constants of the f32 trace (used for roofline accounting). */
+const char* gl_version(void);
+gl_status gl_flop_constants(double* rhs_flops, double* attempt_flops);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GRAVLUME_H */

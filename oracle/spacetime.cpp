@@ -21,7 +21,9 @@ bool finite(double v) { return std::isfinite(v); }
 // the sign of its largest nonzero component, which makes the extremality
 // class exact. The value is rounded to within one ulp (the reference rounds
 // half-even; both agree to the fixture tolerance of 2e-13 on r_+).
-// Exact for |M|, |a|, |Q| in [2^-480, 2^480] where two-product is exact.
+// Inputs are first scaled by a power of two so the largest magnitude lies in
+// [1, 2); the class is exact unless a value is below 2^-480 times the
+// largest one.
 struct Expansion {
   std::vector<double> e;  // increasing magnitude, nonoverlapping
   void grow(double b) {
@@ -52,7 +54,16 @@ struct Expansion {
   }
 };
 
-Expansion discriminant(double mass, double spin, double charge) {
+// The discriminant of (M, a, Q) scaled by 2^-k so that the largest
+// magnitude lies in [1, 2): power-of-two scaling is exact, keeps every square
+// finite and preserves the sign. *scale_out receives 2^k.
+Expansion discriminant(double mass, double spin, double charge, double* scale_out = nullptr) {
+  double big = std::fmax(std::fabs(mass), std::fmax(std::fabs(spin), std::fabs(charge)));
+  int k = big > 0.0 ? std::ilogb(big) : 0;
+  mass = std::ldexp(mass, -k);
+  spin = std::ldexp(spin, -k);
+  charge = std::ldexp(charge, -k);
+  if (scale_out) *scale_out = std::ldexp(1.0, k);
   Expansion ex;
   auto add_square = [&ex](double v, double sign) {
     double hi = v * v;
@@ -151,8 +162,9 @@ int make_spacetime(double mass, double spin, double charge, double chart, Spacet
     st.outer_horizon = mass;
   } else {
     st.has_horizon = true;
-    double d = discriminant(mass, spin, charge).value();
-    st.outer_horizon = mass + std::sqrt(d);
+    double scale = 1.0;
+    double d = discriminant(mass, spin, charge, &scale).value();
+    st.outer_horizon = (std::ldexp(mass, -std::ilogb(scale)) + std::sqrt(d)) * scale;
   }
   if (st.has_horizon && !finite(st.outer_horizon)) return 1;
   *out = st;

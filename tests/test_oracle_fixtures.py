@@ -1,0 +1,168 @@
+"""The oracle against the reference's own golden fixtures (fixtures/v1, v2)
+and its fixture tests (crates/gravlume-reference/tests/fixture_tests.rs)."""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+FIX = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_fixtures.json")))
+
+
+def _angle(a, b):
+    a, b = np.asarray(a, dtype=float), np.asarray(b, dtype=float)
+    return math.atan2(np.linalg.norm(np.cross(a, b)), float(a @ b))
+
+
+def _geodesic(O, name, policy):
+    f = FIX[name]
+    st = O.spacetime(1.0, 0.0, 0.0, 1.0)
+    y = [float(v) for v in f["position_txyz"]] + [float(v) for v in f["momentum_covariant"]]
+    esc = f["escape_radius"]
+    ev = O.events(float(esc)) if esc is not None else O.events(None, has_escape=False)
+    return f, st, y, O.trace(st, O.policy(policy), ev, y, float(f["affine_direction"]))
+
+
+@pytest.mark.parametrize("policy", ["regular", "strict"])
+@pytest.mark.parametrize("name", ["schwarzschild_scatter_b6", "schwarzschild_scatter_near_critical"])
+def test_schwarzschild_scatter_fixtures(oracle, name, policy):
+    # fixture_tests.rs:186-206, 341-349 (ExpectedOutcome::accepts, fixture.rs:230-259)
+    if name.endswith("near_critical") and policy == "regular":
+        pytest.skip("the reference runs the near-critical pair under strict-v1 only")
+    f, _, _, out = _geodesic(oracle, name, policy)
+    assert out.termination == oracle.TERM_ESCAPE
+    assert out.has_turning_radius
+    assert abs(out.turning_radius - float(f["expected"]["turning_radius"])) <= float(f["tolerance"]["turning_radius"])
+    assert abs(out.azimuth_advance - float(f["expected"]["azimuth_advance"])) <= float(f["tolerance"]["azimuth_advance"])
+    assert out.candidates == 8
+    assert out.event_bracket <= oracle.policy(policy).event_affine_tol
+
+
+def test_schwarzschild_capture_fixture(oracle):
+    f, _, _, out = _geodesic(oracle, "schwarzschild_capture_near_critical", "strict")
+    assert out.termination == oracle.TERM_HORIZON
+    assert not out.has_turning_radius
+    r = math.sqrt(out.state[1] ** 2 + out.state[2] ** 2 + out.state[3] ** 2)
+    assert abs(r - 2.0) <= float(f["tolerance"]["event_radius"])
+    assert abs(out.azimuth_advance - float(f["expected"]["azimuth_advance"])) <= float(f["tolerance"]["azimuth_advance"])
+
+
+def test_b6_regular_null_drift_and_residual(oracle):
+    # fixture_tests.rs:192-199
+    _, _, _, out = _geodesic(oracle, "schwarzschild_scatter_b6", "regular")
+    assert out.event_residual <= 5e-12
+    assert out.drift_null < 5e-9
+
+
+def test_regular_vs_strict_comparison_gates(oracle):
+    # ReferenceComparison::baseline_v1 gates (comparison.rs:66-145)
+    _, _, _, reg = _geodesic(oracle, "schwarzschild_scatter_b6", "regular")
+    _, _, _, sti = _geodesic(oracle, "schwarzschild_scatter_b6", "strict")
+    assert reg.termination == sti.termination
+    ang = _angle(reg.escape_direction, sti.escape_direction)
+    assert ang <= 2e-9
+    assert abs(reg.travel_time - sti.travel_time) <= 2e-8
+    pos = math.dist(reg.state[1:4], sti.state[1:4])
+    assert pos <= 2e-9
+    for o in (reg, sti):
+        assert max(o.drift_null, o.drift_energy, o.drift_lz, o.drift_carter) <= 5e-9
+
+
+def test_turning_radius_forward_backward_symmetry(oracle):
+    # fixture_tests.rs:295-314
+    f, st, y, fwd = _geodesic(oracle, "schwarzschild_scatter_b6", "strict")
+    ev = oracle.events(50.0)
+    back = oracle.trace(st, oracle.policy("strict"), ev, list(fwd.state), -1.0)
+    assert back.termination == oracle.TERM_ESCAPE
+    assert abs(back.turning_radius - fwd.turning_radius) <= 5e-11
+
+
+def test_travel_time_independent_of_time_origin(oracle):
+    # fixture_tests.rs:316-339
+    f, st, y, base = _geodesic(oracle, "schwarzschild_scatter_b6", "regular")
+    y2 = list(y)
+    y2[0] = 1.0e300
+    shifted = oracle.trace(st, oracle.policy("regular"), oracle.events(50.0), y2, 1.0)
+    assert shifted.termination == base.termination
+    assert shifted.travel_time > 0.0
+    assert abs(shifted.travel_time - base.travel_time) <= 2e-8
+
+
+def test_ordered_batch_preserves_input_order(oracle):
+    # fixture_tests.rs:351-374 (Rayon order), plus the 256-thread limit (batch.rs:7,19-29)
+    f, st, y, _ = _geodesic(oracle, "schwarzschild_scatter_b6", "regular")
+    y_capture = [0, 50, 0, 0, -1.0, -0.99458801928403951714, 0.10390304845413263761, 0]
+    states = np.array([y, y_capture, y], dtype=np.float64)
+    outs = oracle.trace_ordered(st, oracle.policy("regular"), oracle.events(50.0), states,
+                                np.ones(3), threads=2)
+    assert [o.termination for o in outs] == [2, 1, 2]
+    with pytest.raises(ValueError):
+        oracle.trace_ordered(st, oracle.policy("regular"), oracle.events(50.0), states,
+                             np.ones(3), threads=257)
+
+
+def _kerr_observation(oracle):
+    f = FIX["default_kerr_observation"]
+    o = f["observer"]
+    xyz = oracle.oblate_to_cartesian(0.8, 1.0, float(o["oblate_radius"]), float(o["polar_angle"]), 0.0)
+    vp = f["viewport"]
+    obs = oracle.observation(1.0, 0.8, 0.0, 1.0, [0.0, *xyz], [0, 0, 0, 0], [0, 0, 1], 1.0,
+                             vp["width"], vp["height"], float(vp["vertical_fov"]))
+    return f, obs
+
+
+def test_default_kerr_observation_fixture(oracle):
+    # validate_observation_expected (fixture/v1.rs:575-620)
+    f, obs = _kerr_observation(oracle)
+    tol = float(f["tolerance"]["radius_abs"])
+    exp = [float(v) for v in f["expected"]["observer_cartesian_xyz"]]
+    assert all(abs(obs.event[1 + i] - exp[i]) <= tol for i in range(3))
+    assert abs(obs.st.outer_horizon - 1.6) <= tol
+    assert abs(obs.g_tt - float(f["expected"]["observer_g_tt"])) <= tol
+    assert obs.gram_residual <= float(f["tolerance"]["frame_gram_max_abs"])
+    s = oracle.sight_plane(obs, 640, 360)
+    _, _, freq, null = oracle.initial_ray(obs, *s)
+    assert null <= float(f["tolerance"]["initial_null_normalized_abs"])
+    assert abs(freq - 1.0) <= 32 * np.finfo(float).eps
+
+
+def test_observation_traces_backward_and_escapes(oracle):
+    # fixture_tests.rs:376-429
+    f, obs = _kerr_observation(oracle)
+    s = oracle.sight_plane(obs, 0, 0)
+    outs, status = oracle.observation_trace_ordered(obs, oracle.policy("regular"),
+                                                    oracle.events(200.0), np.array([s]))
+    o = outs[0]
+    assert status[0] == 0 and o.termination == oracle.TERM_ESCAPE
+    assert o.affine < 0.0 and o.travel_time > 0.0
+    d = np.array(o.escape_direction)
+    assert abs(d @ d - 1.0) <= 8 * np.finfo(float).eps
+    assert np.dot(np.array(o.state[1:4]), d) > 0.0
+    outs2, _ = oracle.observation_trace_ordered(obs, oracle.policy("strict"), oracle.events(200.0),
+                                                np.array([s]))
+    ang = _angle(d, outs2[0].escape_direction)
+    assert ang <= 2e-9 and abs(o.travel_time - outs2[0].travel_time) <= 2e-8
+
+
+@pytest.mark.parametrize("policy", ["regular", "strict"])
+def test_kerr_surface_observable_fixture(oracle, policy):
+    # surface_observable_tests.rs:157-218 with ExpectedSurfaceOutcome::accepts (fixture.rs:291-325)
+    f = FIX["kerr_surface_observable"]
+    _, obs = _kerr_observation(oracle)
+    s = oracle.sight_plane(obs, 640, 16)
+    em = oracle.Emitter()
+    em.intensity_at_six = 1.0
+    outs, status = oracle.observation_trace_ordered(obs, oracle.policy(policy),
+                                                    oracle.events(200.0, (6.0, 20.0)),
+                                                    np.array([s]), em)
+    o = outs[0]
+    e, t = f["expected"], f["tolerance"]
+    assert o.termination == oracle.TERM_SURFACE and o.surface_status == 0
+    assert abs(o.source_radius - float(e["source_radius"])) <= float(t["source_coordinate_abs"])
+    dphi = (o.source_azimuth - float(e["source_azimuth"]) + math.pi) % (2 * math.pi) - math.pi
+    assert abs(dphi) <= float(t["source_azimuth_abs"])
+    assert abs(o.frequency_ratio / float(e["frequency_ratio"]) - 1) <= float(t["frequency_ratio_rel"])
+    assert abs(o.travel_time - float(e["travel_time"])) <= float(t["travel_time_abs"])
+    assert abs(o.emitted_intensity - float(e["emitted_bolometric_intensity"])) <= float(t["bolometric_intensity_abs"])
+    assert abs(o.observed_intensity - float(e["observed_bolometric_intensity"])) <= float(t["bolometric_intensity_abs"])
