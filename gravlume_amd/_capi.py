@@ -79,7 +79,8 @@ class RayOutputsC(C.Structure):
 
 class PersistentConfig(C.Structure):
     _fields_ = [("block_size", C.c_uint32), ("blocks_per_cu", C.c_uint32),
-                ("refill_interval", C.c_uint32), ("refill_threshold", C.c_uint32)]
+                ("refill_interval", C.c_uint32), ("refill_threshold", C.c_uint32),
+                ("waves_per_simd", C.c_uint32)]
 
 
 class QueueStats(C.Structure):

@@ -191,6 +191,12 @@ __device__ __noinline__ void write_ray(const RayT<float>& R, const OutPlanes& o,
   }
 }
 
+// Null re-projection on a schedule every lane of a wavefront shares: the
+// f64 branch runs for the whole wave at once instead of once per lane.
+__device__ __forceinline__ bool reproject_now(const SceneT<float>& sc, uint32_t wave_step) {
+  return sc.reproject_interval != 0u && (wave_step % sc.reproject_interval) == 0u;
+}
+
 __device__ __forceinline__ bool wants_defer(const SceneT<float>& sc, const RayT<float>& R) {
   return (sc.options & kOptF64Horizon) && sc.r_horizon > 0.0f && R.r < 1.05f * sc.r_horizon;
 }
@@ -224,9 +230,9 @@ __global__ void __launch_bounds__(64) k_plain_image(const SceneDev* __restrict__
   uint32_t idx = py * s.width + px;
   RayT<float> R;
   init_image_ray(s, sc, px, py, R);
-  while (R.term == kTermActive) {
+  for (uint32_t it = 1; R.term == kTermActive; ++it) {
     if (wants_defer(sc, R) && try_defer(df, R, 0u, idx)) return;
-    step<float>(sc, R, s.omega);
+    step<float>(sc, R, s.omega, reproject_now(sc, it));
   }
   write_ray(R, fsc.out, idx, s, map);
 }
@@ -243,9 +249,9 @@ __global__ void __launch_bounds__(64) k_geodesic_batch(const SceneDev* __restric
   if (i >= n) return;
   RayT<float> R;
   init_state_ray(s, sc, states + 8 * size_t(i), dirs[i] >= 0.0f ? 1.0f : -1.0f, R);
-  while (R.term == kTermActive) {
+  for (uint32_t it = 1; R.term == kTermActive; ++it) {
     if (wants_defer(sc, R) && try_defer(df, R, 0u, i)) return;
-    step<float>(sc, R, s.omega);
+    step<float>(sc, R, s.omega, reproject_now(sc, it));
   }
   write_ray(R, fsc.out, i, s, map);
 }
@@ -260,8 +266,10 @@ __device__ __forceinline__ uint32_t find_frame_scene(const FrameScene* frame, ui
   return lo;
 }
 
-template <bool kSingle>
-__global__ void __launch_bounds__(256) k_persistent(const SceneDev* __restrict__ scenes,
+// kWaves: minimum waves per SIMD the register allocation must allow
+// (the second __launch_bounds__ argument); the autotuner picks it.
+template <bool kSingle, int kWaves>
+__global__ void __launch_bounds__(256, kWaves) k_persistent(const SceneDev* __restrict__ scenes,
                                                     const FrameScene* __restrict__ frame,
                                                     uint32_t n_frame, QueueDev* queue, MapView map,
                                                     DeferBuf df, uint32_t interval,
@@ -275,6 +283,7 @@ __global__ void __launch_bounds__(256) k_persistent(const SceneDev* __restrict__
   SceneT<float> sc;
   if (kSingle) sc = scene_f32(scenes[frame[0].scene]);
   unsigned long long attempts = 0, slots = 0, refills = 0;
+  uint32_t wave_steps = 0;  // wave-uniform attempt counter
   bool drained = false;  // wave-uniform: the queue has no items left
   for (;;) {
     if (!drained) {
@@ -319,12 +328,13 @@ __global__ void __launch_bounds__(256) k_persistent(const SceneDev* __restrict__
       unsigned long long act = __ballot(active);
       attempts += __popcll(act);
       slots += 64;
+      ++wave_steps;
       if (active) {
         const FrameScene& f = frame[fs];
         if (wants_defer(sc, R) && try_defer(df, R, fs, idx)) {
           active = false;
         } else {
-          step<float>(sc, R, scenes[f.scene].omega);
+          step<float>(sc, R, scenes[f.scene].omega, reproject_now(sc, wave_steps));
           if (R.term != kTermActive) {
             write_ray(R, f.out, idx, scenes[f.scene], map);
             active = false;
@@ -389,7 +399,7 @@ __global__ void __launch_bounds__(64) k_finish_f64(const SceneDev* __restrict__ 
       R.term = kTermFailure;
       R.flags |= kFlagNonFinite;
     }
-    while (R.term == kTermActive) step<double>(sd, R, double(s.omega));
+    for (uint32_t it = 1; R.term == kTermActive; ++it) step<double>(sd, R, double(s.omega), false);
     RayT<float> W;
     W.x = float(R.x);
     W.y = float(R.y);
@@ -486,11 +496,14 @@ struct gl_trace_plan {
   FrameScene* d_frame = nullptr;
   FrameScene* h_frame = nullptr;  // pinned staging of the frame table
   size_t frame_capacity = 0;
+  std::vector<FrameScene> frame_shadow;  // what d_frame holds (or will hold)
+  hipEvent_t frame_copied = nullptr;     // completion of the last staging copy
+  bool frame_copy_pending = false;
   DeferRecord* d_defer = nullptr;
   unsigned int* d_defer_count = nullptr;
   uint32_t defer_capacity = 0;
   int cu_count = 256;
-  gl_persistent_config pcfg = {256, 2, 8, 16};
+  gl_persistent_config pcfg = {256, 2, 8, 16, 2};
   bool timing = false;
   bool capturing = false;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};  // main start/stop, finisher start/stop
@@ -500,20 +513,60 @@ struct gl_trace_plan {
 struct gl_frame_graph {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  FrameScene* d_table = nullptr;
 };
 
 namespace {
 
 gl_status ensure_frame(gl_trace_plan* p, size_t n) {
   if (n <= p->frame_capacity) return GL_OK;
+  if (p->frame_copy_pending) GL_HIP(hipEventSynchronize(p->frame_copied));
+  p->frame_copy_pending = false;
   if (p->d_frame) (void)hipFree(p->d_frame);
   if (p->h_frame) (void)hipHostFree(p->h_frame);
   p->d_frame = nullptr;
   p->h_frame = nullptr;
+  p->frame_shadow.clear();
   GL_HIP(hipMalloc(&p->d_frame, n * sizeof(FrameScene)));
   GL_HIP(hipHostMalloc(&p->h_frame, n * sizeof(FrameScene), hipHostMallocDefault));
+  if (!p->frame_copied) GL_HIP(hipEventCreateWithFlags(&p->frame_copied, hipEventDisableTiming));
   p->frame_capacity = n;
   return GL_OK;
+}
+
+// Stream-ordered upload of the frame table. The device copy is skipped when
+// it already holds this table; otherwise the pinned staging buffer is reused
+// only after the previous staging copy has executed. Calls on one plan are
+// ordered on one stream.
+gl_status upload_frame(gl_trace_plan* p, const std::vector<FrameScene>& table, hipStream_t st) {
+  gl_status s = ensure_frame(p, table.size());
+  if (s) return s;
+  if (p->frame_shadow.size() == table.size() &&
+      std::memcmp(p->frame_shadow.data(), table.data(), table.size() * sizeof(FrameScene)) == 0)
+    return GL_OK;
+  if (p->frame_copy_pending) GL_HIP(hipEventSynchronize(p->frame_copied));
+  std::memcpy(p->h_frame, table.data(), table.size() * sizeof(FrameScene));
+  GL_HIP(hipMemcpyAsync(p->d_frame, p->h_frame, table.size() * sizeof(FrameScene),
+                        hipMemcpyHostToDevice, st));
+  if (!p->capturing) {
+    GL_HIP(hipEventRecord(p->frame_copied, st));
+    p->frame_copy_pending = true;
+  } else {
+    // A captured copy replays from the staging buffer: keep it unchanged.
+    p->frame_copy_pending = false;
+  }
+  p->frame_shadow = table;
+  return GL_OK;
+}
+
+FrameScene make_frame(uint32_t scene, uint32_t first, uint32_t rays, const gl_ray_outputs& o) {
+  FrameScene f;
+  std::memset(&f, 0, sizeof f);
+  f.scene = scene;
+  f.first_item = first;
+  f.rays = rays;
+  f.out = to_planes(o);
+  return f;
 }
 
 gl_status ensure_defer(gl_trace_plan* p, size_t rays) {
@@ -572,8 +625,11 @@ gl_status launch_finisher(gl_trace_plan* p, hipStream_t st) {
   return GL_OK;
 }
 
-gl_status persistent_frame(gl_trace_plan* p, const uint32_t* scenes, size_t n_scenes,
-                           const gl_ray_outputs* outputs, hipStream_t st) {
+// The frame table of a persistent launch: one FrameScene per traced scene
+// with the prefix sum of its 8x8-tile work items.
+gl_status build_frame_table(gl_trace_plan* p, const uint32_t* scenes, size_t n_scenes,
+                            const gl_ray_outputs* outputs, std::vector<FrameScene>* table,
+                            unsigned long long* total_items, size_t* total_rays) {
   std::vector<uint32_t> ids;
   if (scenes) {
     ids.assign(scenes, scenes + n_scenes);
@@ -582,32 +638,50 @@ gl_status persistent_frame(gl_trace_plan* p, const uint32_t* scenes, size_t n_sc
   }
   if (ids.empty()) return fail(GL_ERR_INVALID_ARGUMENT, "no scenes to trace");
   if (!outputs) return fail(GL_ERR_INVALID_ARGUMENT, "outputs must not be NULL");
-  gl_status s = ensure_frame(p, ids.size());
-  if (s) return s;
+  table->resize(ids.size());
   unsigned long long total = 0;
-  size_t max_rays = 0;
+  size_t rays_sum = 0;
   for (size_t i = 0; i < ids.size(); ++i) {
     if (ids[i] >= p->scenes.size()) return fail(GL_ERR_INVALID_ARGUMENT, "scene index out of range");
     const SceneDev& sd = p->scenes[ids[i]];
     size_t rays = size_t(sd.width) * sd.height;
-    s = check_outputs(&outputs[i], rays);
+    gl_status s = check_outputs(&outputs[i], rays);
     if (s) return s;
     uint32_t tiles_y = (sd.height + 7) / 8;
     unsigned long long items = 64ull * sd.tiles_x * tiles_y;
     if (total + items > 0xffffffffull) return fail(GL_ERR_CAPACITY, "frame exceeds 2^32 work items");
-    p->h_frame[i].scene = ids[i];
-    p->h_frame[i].first_item = uint32_t(total);
-    p->h_frame[i].rays = uint32_t(rays);
-    p->h_frame[i].out = to_planes(outputs[i]);
+    (*table)[i] = make_frame(ids[i], uint32_t(total), uint32_t(rays), outputs[i]);
     total += items;
-    max_rays += rays;
+    rays_sum += rays;
   }
-  if (any_f64_horizon(p)) {
-    s = ensure_defer(p, max_rays);
+  *total_items = total;
+  *total_rays = rays_sum;
+  return GL_OK;
+}
+
+const void* persistent_kernel(bool single, uint32_t waves) {
+  switch (waves) {
+    case 2:
+      return single ? (const void*)k_persistent<true, 2> : (const void*)k_persistent<false, 2>;
+    case 3:
+      return single ? (const void*)k_persistent<true, 3> : (const void*)k_persistent<false, 3>;
+    case 4:
+      return single ? (const void*)k_persistent<true, 4> : (const void*)k_persistent<false, 4>;
+    default:
+      return nullptr;
+  }
+}
+
+// Queue reset, persistent kernel and (when any scene asks for f64 horizon
+// steps) the f64 finisher, all on `st`, reading the frame table d_table.
+gl_status launch_persistent(gl_trace_plan* p, const FrameScene* d_table, size_t n_table,
+                            unsigned long long total, size_t total_rays, hipStream_t st) {
+  gl_status s = GL_OK;
+  bool f64 = any_f64_horizon(p);
+  if (f64) {
+    s = ensure_defer(p, total_rays);
     if (s) return s;
   }
-  GL_HIP(hipMemcpyAsync(p->d_frame, p->h_frame, ids.size() * sizeof(FrameScene),
-                        hipMemcpyHostToDevice, st));
   hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, st, p->d_queue, total,
                      p->d_defer_count);
   GL_HIP(hipGetLastError());
@@ -615,31 +689,50 @@ gl_status persistent_frame(gl_trace_plan* p, const uint32_t* scenes, size_t n_sc
   // Every workgroup must be resident at once: clamp the requested blocks
   // per CU to what the kernel's register budget allows.
   int resident = 0;
-  if (ids.size() == 1)
-    GL_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, k_persistent<true>,
-                                                        int(c.block_size), 0));
-  else
-    GL_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, k_persistent<false>,
-                                                        int(c.block_size), 0));
+  const void* kern = persistent_kernel(n_table == 1, c.waves_per_simd);
+  if (!kern) return fail(GL_ERR_INVALID_ARGUMENT, "waves_per_simd must be 2, 3 or 4");
+  GL_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, kern, int(c.block_size), 0));
   int per_cu = std::max(1, std::min(int(c.blocks_per_cu), resident));
   dim3 grid(std::max(1, p->cu_count * per_cu));
   dim3 block(c.block_size);
   DeferBuf df = defer_buf(p);
-  if (!any_f64_horizon(p)) df.rec = nullptr;
+  if (!f64) df.rec = nullptr;
   p->finisher_timed = false;
   mark(p, 0, st);
-  if (ids.size() == 1)
-    hipLaunchKernelGGL(k_persistent<true>, grid, block, 0, st, p->d_scenes, p->d_frame,
-                       uint32_t(ids.size()), p->d_queue, map_view(p), df, c.refill_interval,
-                       c.refill_threshold);
-  else
-    hipLaunchKernelGGL(k_persistent<false>, grid, block, 0, st, p->d_scenes, p->d_frame,
-                       uint32_t(ids.size()), p->d_queue, map_view(p), df, c.refill_interval,
-                       c.refill_threshold);
+  {
+    const SceneDev* a0 = p->d_scenes;
+    const FrameScene* a1 = d_table;
+    uint32_t a2 = uint32_t(n_table);
+    QueueDev* a3 = p->d_queue;
+    MapView a4 = map_view(p);
+    DeferBuf a5 = df;
+    uint32_t a6 = c.refill_interval, a7 = c.refill_threshold;
+    void* args[] = {&a0, &a1, &a2, &a3, &a4, &a5, &a6, &a7};
+    GL_HIP(hipLaunchKernel(kern, grid, block, args, 0, st));
+  }
   GL_HIP(hipGetLastError());
   mark(p, 1, st);
-  if (df.rec) return launch_finisher(p, st);
+  if (df.rec) {
+    mark(p, 2, st);
+    hipLaunchKernelGGL(k_finish_f64, dim3(std::max(1, p->cu_count * 4)), dim3(64), 0, st,
+                       p->d_scenes, d_table, map_view(p), df);
+    GL_HIP(hipGetLastError());
+    mark(p, 3, st);
+    p->finisher_timed = true;
+  }
   return GL_OK;
+}
+
+gl_status persistent_frame(gl_trace_plan* p, const uint32_t* scenes, size_t n_scenes,
+                           const gl_ray_outputs* outputs, hipStream_t st) {
+  std::vector<FrameScene> table;
+  unsigned long long total = 0;
+  size_t rays = 0;
+  gl_status s = build_frame_table(p, scenes, n_scenes, outputs, &table, &total, &rays);
+  if (s) return s;
+  s = upload_frame(p, table, st);
+  if (s) return s;
+  return launch_persistent(p, p->d_frame, table.size(), total, rays, st);
 }
 
 }  // namespace
@@ -714,6 +807,7 @@ gl_status gl_trace_plan_destroy(gl_trace_plan* p) {
   if (p->h_frame) (void)hipHostFree(p->h_frame);
   if (p->d_defer) (void)hipFree(p->d_defer);
   if (p->d_defer_count) (void)hipFree(p->d_defer_count);
+  if (p->frame_copied) (void)hipEventDestroy(p->frame_copied);
   for (hipEvent_t e : p->ev)
     if (e) (void)hipEventDestroy(e);
   delete p;
@@ -740,18 +834,13 @@ gl_status gl_trace_encode(gl_trace_plan* p, const gl_tile_region* region,
   gl_status s = check_outputs(out, size_t(sd.width) * sd.height);
   if (s) return s;
   hipStream_t st = (hipStream_t)stream;
-  s = ensure_frame(p, 1);
-  if (s) return s;
-  p->h_frame[0].scene = region->scene;
-  p->h_frame[0].first_item = 0;
-  p->h_frame[0].rays = sd.width * sd.height;
-  p->h_frame[0].out = to_planes(*out);
   bool f64 = (sd.options & kOptF64Horizon) != 0;
   if (f64) {
     s = ensure_defer(p, size_t(region->tiles_x) * region->tiles_y * 64);
     if (s) return s;
   }
-  GL_HIP(hipMemcpyAsync(p->d_frame, p->h_frame, sizeof(FrameScene), hipMemcpyHostToDevice, st));
+  s = upload_frame(p, {make_frame(region->scene, 0, sd.width * sd.height, *out)}, st);
+  if (s) return s;
   hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, st, p->d_queue, 0ull, p->d_defer_count);
   DeferBuf df = defer_buf(p);
   if (!f64) df.rec = nullptr;
@@ -782,6 +871,8 @@ gl_status gl_trace_plan_set_persistent_config(gl_trace_plan* p, const gl_persist
     return fail(GL_ERR_INVALID_ARGUMENT, "refill_interval must lie in [1, 4096]");
   if (c->refill_threshold == 0 || c->refill_threshold > 64)
     return fail(GL_ERR_INVALID_ARGUMENT, "refill_threshold must lie in [1, 64]");
+  if (c->waves_per_simd < 2 || c->waves_per_simd > 4)
+    return fail(GL_ERR_INVALID_ARGUMENT, "waves_per_simd must be 2, 3 or 4");
   p->pcfg = *c;
   return GL_OK;
 }
@@ -802,18 +893,13 @@ gl_status gl_geodesic_batch_trace(gl_trace_plan* p, uint32_t scene, const double
   gl_status s = check_outputs(out, n);
   if (s) return s;
   hipStream_t st = (hipStream_t)stream;
-  s = ensure_frame(p, 1);
-  if (s) return s;
-  p->h_frame[0].scene = scene;
-  p->h_frame[0].first_item = 0;
-  p->h_frame[0].rays = uint32_t(n);
-  p->h_frame[0].out = to_planes(*out);
   bool f64 = (p->scenes[scene].options & kOptF64Horizon) != 0;
   if (f64) {
     s = ensure_defer(p, n);
     if (s) return s;
   }
-  GL_HIP(hipMemcpyAsync(p->d_frame, p->h_frame, sizeof(FrameScene), hipMemcpyHostToDevice, st));
+  s = upload_frame(p, {make_frame(scene, 0, uint32_t(n), *out)}, st);
+  if (s) return s;
   hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, st, p->d_queue, 0ull, p->d_defer_count);
   DeferBuf df = defer_buf(p);
   if (!f64) df.rec = nullptr;
@@ -872,31 +958,47 @@ gl_status gl_frame_graph_create(gl_trace_plan* p, const uint32_t* scenes, size_t
   *out_graph = nullptr;
   hipStream_t st = (hipStream_t)stream;
   if (st == nullptr) return fail(GL_ERR_INVALID_ARGUMENT, "graph capture needs a non-default stream");
+  std::vector<FrameScene> table;
+  unsigned long long total = 0;
+  size_t rays = 0;
+  gl_status s = build_frame_table(p, scenes, n_scenes, outputs, &table, &total, &rays);
+  if (s) return s;
+  if (any_f64_horizon(p)) {
+    s = ensure_defer(p, rays);  // allocate before capture
+    if (s) return s;
+  }
   auto* g = new gl_frame_graph();
-  hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+  // The graph owns its frame table, written once before capture.
+  hipError_t e = hipMalloc(&g->d_table, table.size() * sizeof(FrameScene));
+  if (e == hipSuccess)
+    e = hipMemcpy(g->d_table, table.data(), table.size() * sizeof(FrameScene), hipMemcpyHostToDevice);
   if (e != hipSuccess) {
-    delete g;
+    gl_frame_graph_destroy(g);
+    return hip_fail(e, "graph frame table");
+  }
+  e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) {
+    gl_frame_graph_destroy(g);
     return hip_fail(e, "hipStreamBeginCapture");
   }
   p->capturing = true;
-  gl_status s = persistent_frame(p, scenes, n_scenes, outputs, st);
+  s = launch_persistent(p, g->d_table, table.size(), total, rays, st);
   p->capturing = false;
   hipGraph_t graph = nullptr;
   e = hipStreamEndCapture(st, &graph);
   if (s != GL_OK) {
     if (graph) (void)hipGraphDestroy(graph);
-    delete g;
+    gl_frame_graph_destroy(g);
     return s;
   }
   if (e != hipSuccess) {
-    delete g;
+    gl_frame_graph_destroy(g);
     return hip_fail(e, "hipStreamEndCapture");
   }
   g->graph = graph;
   e = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
   if (e != hipSuccess) {
-    (void)hipGraphDestroy(graph);
-    delete g;
+    gl_frame_graph_destroy(g);
     return hip_fail(e, "hipGraphInstantiate");
   }
   *out_graph = g;
@@ -913,6 +1015,7 @@ gl_status gl_frame_graph_destroy(gl_frame_graph* g) {
   if (!g) return GL_OK;
   if (g->exec) (void)hipGraphExecDestroy(g->exec);
   if (g->graph) (void)hipGraphDestroy(g->graph);
+  if (g->d_table) (void)hipFree(g->d_table);
   delete g;
   return GL_OK;
 }

@@ -576,7 +576,7 @@ GL_HD bool surface_observable(U mass, U spin, U charge2, U chart, U omega, U x, 
 // committed; on termination R.term becomes nonzero and (x, p) hold the
 // localized terminal state.
 template <class T>
-GL_HD void step(const SceneT<T>& sc, RayT<T>& R, T omega) {
+GL_HD void step(const SceneT<T>& sc, RayT<T>& R, T omega, bool reproject = false) {
   if (R.accepted >= sc.max_accepted) {
     R.term = kTermStepExhaustion;
     return;
@@ -798,7 +798,7 @@ GL_HD void step(const SceneT<T>& sc, RayT<T>& R, T omega) {
   update_winding<T>(R, xn[0], xn[1]);
   update_drift<T>(sc, R, g, d, xn[0], xn[1], xn[2], pn[0], pn[1], pn[2]);
   R.armed = arm_events<T>(sc, r1, z1, R.armed);
-  if (sc.reproject_interval != 0u && (R.accepted % sc.reproject_interval) == 0u) {
+  if (reproject) {
     reproject_null<T>(sc, R, R.px, R.py, R.pz);
     if (!refresh_derivative<T>(sc, R)) {
       R.term = kTermFailure;

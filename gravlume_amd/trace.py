@@ -166,8 +166,9 @@ class TracePipeline:
                                                          _stream_handle(stream)))
 
     def set_persistent_config(self, block_size=256, blocks_per_cu=2, refill_interval=8,
-                              refill_threshold=16) -> None:
-        c = _capi.PersistentConfig(block_size, blocks_per_cu, refill_interval, refill_threshold)
+                              refill_threshold=16, waves_per_simd=2) -> None:
+        c = _capi.PersistentConfig(block_size, blocks_per_cu, refill_interval, refill_threshold,
+                                   waves_per_simd)
         _capi.check(self._lib.gl_trace_plan_set_persistent_config(self._handle, C.byref(c)))
 
     def persistent_config(self) -> dict:

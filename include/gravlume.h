@@ -170,6 +170,7 @@ typedef struct gl_persistent_config {
   uint32_t blocks_per_cu;    /* resident workgroups per CU */
   uint32_t refill_interval;  /* DP5 attempts between refill checks */
   uint32_t refill_threshold; /* free lanes that trigger a refill */
+  uint32_t waves_per_simd;   /* register budget: 2, 3 or 4 resident waves per SIMD */
 } gl_persistent_config;
 gl_status gl_trace_plan_set_persistent_config(gl_trace_plan* plan,
                                               const gl_persistent_config* config);
