@@ -197,6 +197,10 @@ __device__ __forceinline__ bool reproject_now(const SceneT<float>& sc, uint32_t 
   return sc.reproject_interval != 0u && (wave_step % sc.reproject_interval) == 0u;
 }
 
+// Invariant drifts are sampled every fourth attempt (and at every
+// re-projection and termination), on the same wavefront-uniform clock.
+__device__ __forceinline__ bool drift_now(uint32_t wave_step) { return (wave_step & 3u) == 0u; }
+
 __device__ __forceinline__ bool wants_defer(const SceneT<float>& sc, const RayT<float>& R) {
   return (sc.options & kOptF64Horizon) && sc.r_horizon > 0.0f && R.r < 1.05f * sc.r_horizon;
 }
@@ -232,7 +236,7 @@ __global__ void __launch_bounds__(64) k_plain_image(const SceneDev* __restrict__
   init_image_ray(s, sc, px, py, R);
   for (uint32_t it = 1; R.term == kTermActive; ++it) {
     if (wants_defer(sc, R) && try_defer(df, R, 0u, idx)) return;
-    step<float>(sc, R, s.omega, reproject_now(sc, it));
+    step<float>(sc, R, s.omega, reproject_now(sc, it), drift_now(it));
   }
   write_ray(R, fsc.out, idx, s, map);
 }
@@ -251,7 +255,7 @@ __global__ void __launch_bounds__(64) k_geodesic_batch(const SceneDev* __restric
   init_state_ray(s, sc, states + 8 * size_t(i), dirs[i] >= 0.0f ? 1.0f : -1.0f, R);
   for (uint32_t it = 1; R.term == kTermActive; ++it) {
     if (wants_defer(sc, R) && try_defer(df, R, 0u, i)) return;
-    step<float>(sc, R, s.omega, reproject_now(sc, it));
+    step<float>(sc, R, s.omega, reproject_now(sc, it), drift_now(it));
   }
   write_ray(R, fsc.out, i, s, map);
 }
@@ -334,7 +338,8 @@ __global__ void __launch_bounds__(256, kWaves) k_persistent(const SceneDev* __re
         if (wants_defer(sc, R) && try_defer(df, R, fs, idx)) {
           active = false;
         } else {
-          step<float>(sc, R, scenes[f.scene].omega, reproject_now(sc, wave_steps));
+          step<float>(sc, R, scenes[f.scene].omega, reproject_now(sc, wave_steps),
+                      drift_now(wave_steps));
           if (R.term != kTermActive) {
             write_ray(R, f.out, idx, scenes[f.scene], map);
             active = false;
@@ -377,7 +382,6 @@ __global__ void __launch_bounds__(64) k_finish_f64(const SceneDev* __restrict__ 
     R.ct = 0.0;
     R.lam = double(Rf.lam) - double(Rf.clam);
     R.clam = 0.0;
-    R.prev_az = Rf.prev_az;
     R.lz0 = Rf.lz0;
     R.q0 = Rf.q0;
     R.d_null = Rf.d_null;
@@ -399,7 +403,8 @@ __global__ void __launch_bounds__(64) k_finish_f64(const SceneDev* __restrict__ 
       R.term = kTermFailure;
       R.flags |= kFlagNonFinite;
     }
-    for (uint32_t it = 1; R.term == kTermActive; ++it) step<double>(sd, R, double(s.omega), false);
+    for (uint32_t it = 1; R.term == kTermActive; ++it)
+      step<double>(sd, R, double(s.omega), false, drift_now(it));
     RayT<float> W;
     W.x = float(R.x);
     W.y = float(R.y);
@@ -886,9 +891,10 @@ gl_status gl_trace_plan_get_persistent_config(const gl_trace_plan* p, gl_persist
 gl_status gl_geodesic_batch_trace(gl_trace_plan* p, uint32_t scene, const double* states8,
                                   const float* directions, size_t n, const gl_ray_outputs* out,
                                   void* stream) {
-  if (!p || !states8 || !directions || !out) return fail(GL_ERR_INVALID_ARGUMENT, "plan, states, directions and outputs are required");
+  if (!p || !out) return fail(GL_ERR_INVALID_ARGUMENT, "plan and outputs are required");
   if (scene >= p->scenes.size()) return fail(GL_ERR_INVALID_ARGUMENT, "scene index out of range");
-  if (n == 0) return GL_OK;
+  if (n == 0) return GL_OK;  // an empty batch is a no-op (batch.rs:31-43)
+  if (!states8 || !directions) return fail(GL_ERR_INVALID_ARGUMENT, "states and directions are required");
   if (n > 0xffffffffull) return fail(GL_ERR_CAPACITY, "more than 2^32 requests");
   gl_status s = check_outputs(out, n);
   if (s) return s;

@@ -101,7 +101,6 @@ GL_HD Geom<T> geometry(T x, T y, T z, T mass, T spin, T spin2, T charge2, T s) {
   T r2 = b >= T(0) ? T(0.5) * (b + S) : (T(2) * az2) * gl_rcp<T>(S - b);
   T rinv = gl_rsqrt<T>(r2);
   T r = r2 * rinv;
-  sinv = gl_rcp<T>(S);
   T kinv = gl_rcp<T>(r2 + spin2);
   T num = T(2) * mass * r - charge2;
   T f = num * sinv;
@@ -181,7 +180,7 @@ GL_HD T null_residual(const Deriv<T>& d, T px, T py, T pz, T E) {
   T n = gl_abs<T>(E * d.dt) + gl_abs<T>(px * d.dx) + gl_abs<T>(py * d.dy) +
         gl_abs<T>(pz * d.dz);
   T den = n > T(1) ? n : T(1);
-  return gl_abs<T>(c) / den;
+  return gl_abs<T>(c) * gl_rcp<T>(den);
 }
 
 // Carter constant in the axis-regular Cartesian form (docs/physics.md:318-331

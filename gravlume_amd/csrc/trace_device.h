@@ -178,7 +178,6 @@ struct RayT {
   T h;        // signed next step
   T t, ct;    // signed coordinate-time delta
   T lam, clam;
-  T prev_az;
   T lz0, q0;
   T d_null, d_lz, d_q;
   uint32_t accepted, rejected, consec;
@@ -364,7 +363,6 @@ GL_HD void init_ray(const SceneT<T>& sc, RayT<T>& R, T x, T y, T z, T cx, T cy, 
   R.candidates = 0;
   R.flags = 0;
   R.polar_side = polar_side;
-  R.prev_az = gl_atan2<T>(y, x);
   if (!refresh_derivative<T>(sc, R)) {
     R.term = kTermFailure;
     R.flags |= kFlagNonFinite;
@@ -426,7 +424,7 @@ GL_HD T radius_root(const SC& sc, const T K[7][7], T h, T x0, T y0, T z0, T g0, 
   T th = T(0.5);
   for (int it = 0; it < 24; ++it) {
     T den = ghi - glo;
-    th = den != T(0) ? lo - glo * (hi - lo) / den : T(0.5) * (lo + hi);
+    th = den != T(0) ? lo - glo * (hi - lo) * gl_rcp<T>(den) : T(0.5) * (lo + hi);
     if (!(th > lo && th < hi)) th = T(0.5) * (lo + hi);
     T x, y, z;
     dense_position<T>(K, h, x0, y0, z0, th, x, y, z);
@@ -513,8 +511,8 @@ GL_HD void update_drift(const SceneT<T>& sc, RayT<T>& R, const Geom<T>& g, const
   T nr = null_residual<T>(d, px, py, pz, R.E);
   T lz = x * py - y * px;
   T q = carter<T>(g, x, y, z, px, py, pz, R.E, sc.spin2);
-  T dl = gl_abs<T>(lz - R.lz0) / gl_max<T>(T(1), gl_abs<T>(R.lz0));
-  T dq = gl_abs<T>(q - R.q0) / gl_max<T>(T(1), gl_abs<T>(R.q0));
+  T dl = gl_abs<T>(lz - R.lz0) * gl_rcp<T>(gl_max<T>(T(1), gl_abs<T>(R.lz0)));
+  T dq = gl_abs<T>(q - R.q0) * gl_rcp<T>(gl_max<T>(T(1), gl_abs<T>(R.q0)));
   R.d_null = gl_max<T>(R.d_null, nr);
   R.d_lz = gl_max<T>(R.d_lz, dl);
   R.d_q = gl_max<T>(R.d_q, dq);
@@ -522,15 +520,23 @@ GL_HD void update_drift(const SceneT<T>& sc, RayT<T>& R, const Geom<T>& g, const
     R.flags |= kFlagDrift;
 }
 
-// Signed azimuth winding: a jump of atan2 by more than pi between committed
-// states crosses the branch cut (geodesic.rs:406-418).
+// Signed azimuth winding (geodesic.rs:406-418): the reference counts a jump
+// of atan2(y, x) by more than pi between committed states, i.e. a crossing of
+// the branch cut, the half-plane y = 0, x < 0. Here the crossing is read off
+// the chord between the two committed states without atan2: y changes sign
+// and the chord meets y = 0 at x < 0. Going from y >= 0 to y < 0 there
+// (atan2 jumps from +pi to -pi) adds one turn, the reverse removes one.
+// Chords span at most a fraction of a turn (|h| <= h_max_per_r r), where
+// both tests agree.
 template <class T>
-GL_HD void update_winding(RayT<T>& R, T x, T y) {
-  T az = gl_atan2<T>(y, x);
-  T diff = az - R.prev_az;
-  if (diff > T(3.14159265358979323846)) R.winding -= 1;
-  else if (diff < T(-3.14159265358979323846)) R.winding += 1;
-  R.prev_az = az;
+GL_HD void update_winding(RayT<T>& R, T x0, T y0, T x1, T y1) {
+  const bool up0 = y0 >= T(0), up1 = y1 >= T(0);
+  if (up0 == up1) return;
+  // x at the crossing, scaled by (y0 - y1) to avoid a division
+  T xc = x0 * (y0 - y1) + (x1 - x0) * y0;
+  bool neg = (y0 - y1) > T(0) ? xc < T(0) : xc > T(0);
+  if (!neg) return;
+  R.winding += up0 ? 1 : -1;
 }
 
 // Source anchor (r, phi_s) and frequency ratio g of the prograde Keplerian
@@ -576,7 +582,8 @@ GL_HD bool surface_observable(U mass, U spin, U charge2, U chart, U omega, U x, 
 // committed; on termination R.term becomes nonzero and (x, p) hold the
 // localized terminal state.
 template <class T>
-GL_HD void step(const SceneT<T>& sc, RayT<T>& R, T omega, bool reproject = false) {
+GL_HD void step(const SceneT<T>& sc, RayT<T>& R, T omega, bool reproject = false,
+                 bool drift_now = true) {
   if (R.accepted >= sc.max_accepted) {
     R.term = kTermStepExhaustion;
     return;
@@ -690,7 +697,7 @@ GL_HD void step(const SceneT<T>& sc, RayT<T>& R, T omega, bool reproject = false
   if (R.armed & kEvSing) {
     T m0 = singularity_measure<T>(r0, sc.spin2, z0) - sc.guard;
     T m1 = singularity_measure<T>(r1, sc.spin2, z1) - sc.guard;
-    if (m0 > T(0) && m1 <= T(0)) fr[0] = gl_clamp<T>(m0 / (m0 - m1), T(0), T(1));
+    if (m0 > T(0) && m1 <= T(0)) fr[0] = gl_clamp<T>(m0 * gl_rcp<T>(m0 - m1), T(0), T(1));
   }
   if ((R.armed & kEvHor) && r0 > sc.r_capture && r1 <= sc.r_capture)
     fr[1] = radius_root<T>(sc, K, h, y0[0], y0[1], y0[2], r0 - sc.r_capture, r1 - sc.r_capture,
@@ -756,7 +763,7 @@ GL_HD void step(const SceneT<T>& sc, RayT<T>& R, T omega, bool reproject = false
       if ((v0 < T(0) && vt >= T(0)) || (v0 > T(0) && vt <= T(0))) R.turnings += 1;
     }
     if (zcross && thz < best) R.crossings += 1;
-    update_winding<T>(R, ys[0], ys[1]);
+    update_winding<T>(R, y0[0], y0[1], ys[0], ys[1]);
     if (okl)
       update_drift<T>(sc, R, gl_, dl, ys[0], ys[1], ys[2], ys[3], ys[4], ys[5]);
     else
@@ -795,8 +802,9 @@ GL_HD void step(const SceneT<T>& sc, RayT<T>& R, T omega, bool reproject = false
   for (int c = 0; c < 7; ++c) R.k[c] = K[6][c];
   R.r = r1;
   R.rdot = rd1;
-  update_winding<T>(R, xn[0], xn[1]);
-  update_drift<T>(sc, R, g, d, xn[0], xn[1], xn[2], pn[0], pn[1], pn[2]);
+  update_winding<T>(R, y0[0], y0[1], xn[0], xn[1]);
+  if (reproject || drift_now)
+    update_drift<T>(sc, R, g, d, xn[0], xn[1], xn[2], pn[0], pn[1], pn[2]);
   R.armed = arm_events<T>(sc, r1, z1, R.armed);
   if (reproject) {
     reproject_null<T>(sc, R, R.px, R.py, R.pz);

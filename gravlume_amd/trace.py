@@ -247,9 +247,11 @@ class GeodesicBatch:
         dirs = np.ascontiguousarray(directions, dtype=np.float32).reshape(-1)
         if dirs.shape[0] != n:
             raise ValueError("one direction per state is required")
+        target = TraceTarget(n)
+        if n == 0:
+            return target.to_host()
         d_states = torch.from_numpy(states).cuda()
         d_dirs = torch.from_numpy(dirs).cuda()
-        target = TraceTarget(n)
         out = target.c_struct()
         lib = self.pipeline._lib
         _capi.check(lib.gl_geodesic_batch_trace(self.pipeline.handle, self.scene,

@@ -78,7 +78,8 @@ def _compare(h, A, idx, tol=TOL):
     # branch key on rays whose class agrees
     same = ok & (gt == A["termination"])
     for key, ref in (("radial_turnings", "radial_turnings"),
-                     ("equatorial_crossings", "equatorial_crossings")):
+                     ("equatorial_crossings", "equatorial_crossings"),
+                     ("azimuth_winding", "azimuth_winding")):
         assert np.mean((h[key][idx] == A[ref])[same]) >= 0.995, key
 
 
