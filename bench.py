@@ -9,8 +9,14 @@ one full frame (2,073,600 rays) traced by the persistent kernel; the plain
 one-ray-per-lane kernel is timed on the same frame for the ratio.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+                    [--persistent-config WAVES,INTERVAL,THRESHOLD]
+                    [--copy-calibration-mib M]
 
-Prints one JSON line.
+Prints one JSON line. Kernel durations are the mean over the K timed frames,
+from HIP events the library records on its launch stream around the main
+kernel and the f64 finisher. --persistent-config skips the autotuner (for
+profiling runs); --copy-calibration-mib first copies M MiB device to device,
+the known-size kernel that scales the FETCH_SIZE / WRITE_SIZE counters.
 """
 from __future__ import annotations
 
@@ -31,7 +37,27 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--accuracy-rays", type=int, default=2048)
+    ap.add_argument("--persistent-config", type=str, default=None,
+                    help="WAVES,INTERVAL,THRESHOLD instead of autotuning")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0)
     return ap.parse_args()
+
+
+PROFILES = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles")
+
+
+def measured_traffic(path=os.path.join(PROFILES, "k_persistent_tiles_1080p_traffic.md")):
+    """HBM bytes per launch of the persistent kernel on this workload, from the
+    committed FETCH_SIZE / WRITE_SIZE summary (scaled by the copy-kernel
+    calibration recorded there), or None if the summary is absent."""
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("hbm_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
 
 
 def angle(a, b):
@@ -118,29 +144,38 @@ def main():
     rhs_flops, attempt_flops = g.flop_constants()
     target = pipe.create_target(0)
     plain_target = pipe.create_target(0)
-    tuned = autotune(pipe, [target])
+    if args.copy_calibration_mib > 0:
+        n = args.copy_calibration_mib << 20
+        src = torch.ones(n // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+    if args.persistent_config:
+        w, i, t = (int(v) for v in args.persistent_config.split(","))
+        pipe.set_persistent_config(block_size=256, blocks_per_cu=w, refill_interval=i,
+                                   refill_threshold=t, waves_per_simd=w)
+        tuned = {"config": pipe.persistent_config()}
+    else:
+        tuned = autotune(pipe, [target])
     for _ in range(args.warmup):
         pipe.encode_persistent([target])
         pipe.encode_batch(plain_target)
     torch.cuda.synchronize()
 
     def timed(fn):
-        # wall clock over exactly `steps` frames, bracketed by synchronization
+        # wall clock over exactly `steps` frames, bracketed by synchronization;
+        # the library's events give the mean kernel durations of those frames
         torch.cuda.synchronize()
+        pipe.kernel_ms()  # start a new count
         t0 = time.perf_counter()
         for _ in range(args.steps):
             fn()
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
-        # kernel durations from the HIP events the library records around the
-        # main kernel and the f64 finisher, sampled frame by frame
-        main_ms, fin_ms = [], []
-        for _ in range(max(3, min(args.steps, 10))):
-            fn()
-            m, f = pipe.kernel_ms()
-            main_ms.append(m)
-            fin_ms.append(f)
-        return wall, float(np.mean(main_ms)), float(np.mean(fin_ms))
+        main_ms, fin_ms, launches = pipe.kernel_ms()
+        assert launches == args.steps, (launches, args.steps)
+        return wall, main_ms, fin_ms
 
     t_pers, k_pers, f_pers = timed(lambda: pipe.encode_persistent([target]))
     stats = pipe.queue_stats()
@@ -186,7 +221,7 @@ def main():
         "terminations": {int(k): int(v) for k, v in zip(*np.unique(host["termination"], return_counts=True))},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "frac_of_nonpacked_fp32": achieved / 78.6,
-                     "traffic": None,
+                     "traffic": measured_traffic(),
                      "flops_per_launch": flops, "flops_per_attempt": attempt_flops,
                      "flops_per_rhs": rhs_flops, "attempts_per_launch": attempts},
         "cpu_baseline": None,

@@ -103,7 +103,8 @@ SIGNATURES = {
                                           C.POINTER(RayOutputsC), _P]),
     "gl_trace_plan_queue_stats": (C.c_int, [_P, C.POINTER(QueueStats), _P]),
     "gl_trace_plan_enable_timing": (C.c_int, [_P, C.c_int]),
-    "gl_trace_plan_kernel_ms": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "gl_trace_plan_kernel_ms": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                         C.POINTER(C.c_uint64)]),
     "gl_frame_graph_create": (C.c_int, [_P, _P, C.c_size_t, C.POINTER(RayOutputsC), _P,
                                         C.POINTER(_P)]),
     "gl_frame_graph_launch": (C.c_int, [_P, _P]),

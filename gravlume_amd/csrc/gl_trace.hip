@@ -53,12 +53,17 @@ __device__ __forceinline__ void image_pixel(uint32_t item, uint32_t tiles_x, uin
   py = (tile / tiles_x) * 8u + (t >> 3);
 }
 
-// Initial backward ray of pixel (px, py): sight direction from the tetrad,
-// future-directed momentum p = omega (u + n), lowered with the Kerr-Schild
-// metric at the observer (scene.rs:211-271, view.rs:107-123), then projected
-// onto the null cone in f64.
-__device__ __noinline__ void init_image_ray(const SceneDev& s, const SceneT<float>& sc,
-                                            uint32_t px, uint32_t py, RayT<float>& R) {
+// Sight momentum of pixel (px, py): direction from the tetrad, future-directed
+// p = omega (u + n), lowered with the Kerr-Schild metric at the observer
+// (scene.rs:211-271, view.rs:107-123), then projected onto the null cone in
+// f64. Out of line and returned in registers (four dwords), so the ray state
+// of the calling kernel never has its address taken and stays in VGPRs.
+struct Momentum {
+  float px, py, pz, E;
+};
+
+__device__ __noinline__ Momentum image_momentum(const SceneDev& s, uint32_t px, uint32_t py) {
+  const SceneT<float> sc = scene_f32(s);
   float nx = 2.0f * (float(px) + s.sub_x) / float(s.width) - 1.0f;
   float ny = 1.0f - 2.0f * (float(py) + s.sub_y) / float(s.height);
   float sx = s.aspect * s.tan_half_fov * nx;
@@ -74,16 +79,26 @@ __device__ __noinline__ void init_image_ray(const SceneDev& s, const SceneT<floa
   float lp = pu[0] + g.lx * pu[1] + g.ly * pu[2] + g.lz * pu[3];
   float fl = g.f * lp;
   float p_t = -pu[0] + fl;
-  float pxv = pu[1] + fl * g.lx, pyv = pu[2] + fl * g.ly, pzv = pu[3] + fl * g.lz;
-  float E = -p_t;
-  R.x = s.obs_x;
-  R.y = s.obs_y;
-  R.z = s.obs_z;
-  R.cx = R.cy = R.cz = 0.0f;
-  R.E = E;
-  reproject_null<float>(sc, R, pxv, pyv, pzv);
-  init_ray<float>(sc, R, s.obs_x, s.obs_y, s.obs_z, 0.0f, 0.0f, 0.0f, pxv, pyv, pzv, E, -1.0f,
-                  s.polar_side);
+  Momentum m;
+  m.px = pu[1] + fl * g.lx;
+  m.py = pu[2] + fl * g.ly;
+  m.pz = pu[3] + fl * g.lz;
+  m.E = -p_t;
+  RayT<float> at;  // the observer event, for the projection
+  at.x = s.obs_x;
+  at.y = s.obs_y;
+  at.z = s.obs_z;
+  at.cx = at.cy = at.cz = 0.0f;
+  at.E = m.E;
+  reproject_null<float>(sc, at, m.px, m.py, m.pz);
+  return m;
+}
+
+__device__ __forceinline__ void init_image_ray(const SceneDev& s, const SceneT<float>& sc,
+                                               uint32_t px, uint32_t py, RayT<float>& R) {
+  Momentum m = image_momentum(s, px, py);
+  init_ray<float>(sc, R, s.obs_x, s.obs_y, s.obs_z, 0.0f, 0.0f, 0.0f, m.px, m.py, m.pz, m.E,
+                  -1.0f, s.polar_side);
   R.h = -s.h_init;
 }
 
@@ -100,71 +115,54 @@ __device__ void init_state_ray(const SceneDev& s, const SceneT<float>& sc, const
 // Terminal observables from the localized state: the unit traversal
 // direction at an escape (geodesic.rs:480-497), or the source anchor and
 // frequency ratio of the prograde emitter in f64 at a surface hit
-// (surface.rs:75-146).
-__device__ void terminal_observables(const RayT<float>& R, const SceneDev& s, float out[3],
-                                     uint32_t& flags) {
-  out[0] = out[1] = out[2] = 0.0f;
-  if (R.term == kTermEscape) {
+// (surface.rs:75-146); then the packed status, the observable planes and the
+// radiance. Out of line (it runs once per ray and its f64 temporaries stay
+// out of the trace loop's register allocation) and called with scalars only,
+// so no ray state goes through memory.
+__device__ __noinline__ void write_terminal(const OutPlanes& o, uint32_t idx, const SceneDev& s,
+                                            MapView map, uint32_t term, uint32_t candidates,
+                                            uint32_t flags, float dir, float x, float y, float z,
+                                            float cx, float cy, float cz, float px, float py,
+                                            float pz, float E) {
+  const unsigned long long n = o.count;
+  float ob[3] = {0.0f, 0.0f, 0.0f};
+  if (term == kTermEscape) {
     const SceneT<float> sc = scene_f32(s);
     Deriv<float> d;
     Geom<float> g;
-    if (eval<float>(sc, R.x, R.y, R.z, R.px, R.py, R.pz, R.E, d, g)) {
-      float vx = R.dir * d.dx, vy = R.dir * d.dy, vz = R.dir * d.dz;
+    if (eval<float>(sc, x, y, z, px, py, pz, E, d, g)) {
+      float vx = dir * d.dx, vy = dir * d.dy, vz = dir * d.dz;
       float inv = 1.0f / sqrtf(vx * vx + vy * vy + vz * vz);
-      out[0] = vx * inv;
-      out[1] = vy * inv;
-      out[2] = vz * inv;
+      ob[0] = vx * inv;
+      ob[1] = vy * inv;
+      ob[2] = vz * inv;
     } else {
       flags |= kFlagNonFinite;
     }
-  } else if (R.term == kTermSurface) {
-    double x = double(R.x) - double(R.cx), y = double(R.y) - double(R.cy),
-           z = double(R.z) - double(R.cz);
-    Geom<double> g = geometry<double>(x, y, z, double(s.mass), s.spin_d, s.spin_d * s.spin_d,
+  } else if (term == kTermSurface) {
+    double xd = double(x) - double(cx), yd = double(y) - double(cy), zd = double(z) - double(cz);
+    Geom<double> g = geometry<double>(xd, yd, zd, double(s.mass), s.spin_d, s.spin_d * s.spin_d,
                                       s.charge2_d, double(s.chart));
-    double lz = x * double(R.py) - y * double(R.px);
-    double o[3];
+    double lz = xd * double(py) - yd * double(px);
+    double o3[3];
     if (g.ok && surface_observable<double>(double(s.mass), s.spin_d, s.charge2_d, double(s.chart),
-                                           double(s.omega), x, y, g.r, double(R.E), lz, o)) {
-      out[0] = float(o[0]);
-      out[1] = float(o[1]);
-      out[2] = float(o[2]);
+                                           double(s.omega), xd, yd, g.r, double(E), lz, o3)) {
+      ob[0] = float(o3[0]);
+      ob[1] = float(o3[1]);
+      ob[2] = float(o3[2]);
     } else {
       flags |= kFlagDenominator;
     }
   }
-}
-
-// Not inlined: it runs once per ray, and keeping it out of line keeps its
-// temporaries out of the trace loop's register allocation.
-__device__ __noinline__ void write_ray(const RayT<float>& R, const OutPlanes& o, uint32_t idx,
-                                       const SceneDev& s, const MapView& map) {
-  const unsigned long long n = o.count;
-  if (idx >= n) return;
-  float ob[3];
-  uint32_t flags = R.flags;
-  terminal_observables(R, s, ob, flags);
-  if (o.status) o.status[idx] = (R.term & 15u) | ((R.candidates & 15u) << 4) | ((flags & 255u) << 8);
-  if (o.steps) o.steps[idx] = R.accepted;
-  if (o.rejected) o.rejected[idx] = R.rejected;
+  if (o.status) o.status[idx] = (term & 15u) | ((candidates & 15u) << 4) | ((flags & 255u) << 8);
   if (o.observable) {
     o.observable[idx] = ob[0];
     o.observable[n + idx] = ob[1];
     o.observable[2 * n + idx] = ob[2];
   }
-  if (o.travel_time) o.travel_time[idx] = R.dir * (R.t - R.ct);
-  if (o.affine) o.affine[idx] = R.lam - R.clam;
-  if (o.drift) {
-    o.drift[idx] = R.d_null;
-    o.drift[n + idx] = R.d_lz;
-    o.drift[2 * n + idx] = R.d_q;
-  }
-  if (o.branch)
-    o.branch[idx] = (min(R.turnings, 255u)) | (min(R.crossings, 255u) << 8) |
-                    ((uint32_t(R.winding) & 255u) << 16) | ((R.polar_side & 3u) << 24);
   if (o.radiance) {
     float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
-    if (R.term == kTermEscape && map.rgb && (s.options & kOptEscapeMap)) {
+    if (term == kTermEscape && map.rgb && (s.options & kOptEscapeMap)) {
       // equirectangular lookup of the escape direction
       float lon = atan2f(ob[1], ob[0]);
       float colat = acosf(fminf(fmaxf(ob[2], -1.0f), 1.0f));
@@ -177,7 +175,7 @@ __device__ __noinline__ void write_ray(const RayT<float>& R, const OutPlanes& o,
       c0 = map.rgb[t];
       c1 = map.rgb[plane + t];
       c2 = map.rgb[2 * plane + t];
-    } else if (R.term == kTermSurface && ob[0] > 0.0f) {
+    } else if (term == kTermSurface && ob[0] > 0.0f) {
       float r = ob[0], g = ob[2];
       float ratio = r / s.disk_in;
       c0 = g * s.temperature_inner * powf(ratio, -0.75f);  // observed temperature
@@ -189,6 +187,38 @@ __device__ __noinline__ void write_ray(const RayT<float>& R, const OutPlanes& o,
     o.radiance[n + idx] = c1;
     o.radiance[2 * n + idx] = c2;
   }
+}
+
+// All output planes of one finished ray.
+__device__ __forceinline__ void write_ray(const RayT<float>& R, const OutPlanes& o, uint32_t idx,
+                                          const SceneDev& s, const MapView& map) {
+  const unsigned long long n = o.count;
+  if (idx >= n) return;
+  if (o.steps) o.steps[idx] = R.accepted;
+  if (o.rejected) o.rejected[idx] = R.rejected;
+  if (o.travel_time) o.travel_time[idx] = R.dir * (R.t - R.ct);
+  if (o.affine) o.affine[idx] = R.lam - R.clam;
+  if (o.drift) {
+    o.drift[idx] = R.d_null;
+    o.drift[n + idx] = R.d_lz;
+    o.drift[2 * n + idx] = R.d_q;
+  }
+  if (o.branch)
+    o.branch[idx] = (min(R.turnings, 255u)) | (min(R.crossings, 255u) << 8) |
+                    ((uint32_t(R.winding) & 255u) << 16) | ((R.polar_side & 3u) << 24);
+  write_terminal(o, idx, s, map, R.term, R.candidates, R.flags, R.dir, R.x, R.y, R.z, R.cx, R.cy,
+                 R.cz, R.px, R.py, R.pz, R.E);
+}
+
+// Scene constants of the trace loop held in VGPRs: as wave-uniform values
+// they would occupy some twenty SGPRs next to the loop's exec masks, and the
+// scalar file then spills to VGPR lanes with a readlane per use.
+__device__ __forceinline__ void hold_in_vgprs(SceneT<float>& sc) {
+  asm volatile("" : "+v"(sc.mass), "+v"(sc.spin), "+v"(sc.spin2), "+v"(sc.charge2), "+v"(sc.chart));
+  asm volatile("" : "+v"(sc.r_capture), "+v"(sc.r_horizon), "+v"(sc.r_escape), "+v"(sc.disk_in),
+               "+v"(sc.disk_out));
+  asm volatile("" : "+v"(sc.band), "+v"(sc.rtol), "+v"(sc.atol), "+v"(sc.h_min), "+v"(sc.h_max),
+               "+v"(sc.h_max_per_r), "+v"(sc.tie_tol), "+v"(sc.drift_limit), "+v"(sc.guard));
 }
 
 // Null re-projection on a schedule every lane of a wavefront shares: the
@@ -223,9 +253,10 @@ __global__ void __launch_bounds__(64) k_plain_image(const SceneDev* __restrict__
                                                     const FrameScene* __restrict__ frame,
                                                     uint32_t tile_x0, uint32_t tile_y0,
                                                     uint32_t tiles_x, MapView map, DeferBuf df) {
-  const FrameScene fsc = frame[0];
+  const FrameScene& fsc = frame[0];
   const SceneDev& s = scenes[fsc.scene];
-  const SceneT<float> sc = scene_f32(s);
+  SceneT<float> sc = scene_f32(s);
+  hold_in_vgprs(sc);
   uint32_t tx = tile_x0 + blockIdx.x % tiles_x;
   uint32_t ty = tile_y0 + blockIdx.x / tiles_x;
   uint32_t px = tx * 8u + (threadIdx.x & 7u);
@@ -246,7 +277,7 @@ __global__ void __launch_bounds__(64) k_geodesic_batch(const SceneDev* __restric
                                                        const double* __restrict__ states,
                                                        const float* __restrict__ dirs,
                                                        uint32_t n, MapView map, DeferBuf df) {
-  const FrameScene fsc = frame[0];
+  const FrameScene& fsc = frame[0];
   const SceneDev& s = scenes[fsc.scene];
   const SceneT<float> sc = scene_f32(s);
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -285,9 +316,15 @@ __global__ void __launch_bounds__(256, kWaves) k_persistent(const SceneDev* __re
   bool active = false;
   uint32_t fs = 0, idx = 0;
   SceneT<float> sc;
-  if (kSingle) sc = scene_f32(scenes[frame[0].scene]);
-  unsigned long long attempts = 0, slots = 0, refills = 0;
-  uint32_t wave_steps = 0;  // wave-uniform attempt counter
+  float omega = 0.0f;
+  if (kSingle) {
+    sc = scene_f32(scenes[frame[0].scene]);
+    omega = scenes[frame[0].scene].omega;
+    hold_in_vgprs(sc);
+  }
+  uint32_t lane_attempts = 0;  // per lane: DP5 attempts made
+  uint32_t refills = 0;
+  uint32_t wave_steps = 0;  // wave-uniform attempt counter (lane slots / 64)
   bool drained = false;  // wave-uniform: the queue has no items left
   for (;;) {
     if (!drained) {
@@ -307,7 +344,10 @@ __global__ void __launch_bounds__(256, kWaves) k_persistent(const SceneDev* __re
             fs = kSingle ? 0u : find_frame_scene(frame, n_frame, item);
             const FrameScene& f = frame[fs];
             const SceneDev& s = scenes[f.scene];
-            if (!kSingle) sc = scene_f32(s);
+            if (!kSingle) {
+              sc = scene_f32(s);
+              omega = s.omega;
+            }
             uint32_t local = uint32_t(item - f.first_item);
             uint32_t px, py;
             image_pixel(local, s.tiles_x, px, py);
@@ -329,18 +369,15 @@ __global__ void __launch_bounds__(256, kWaves) k_persistent(const SceneDev* __re
       continue;
     }
     for (uint32_t it = 0; it < interval; ++it) {
-      unsigned long long act = __ballot(active);
-      attempts += __popcll(act);
-      slots += 64;
       ++wave_steps;
       if (active) {
-        const FrameScene& f = frame[fs];
+        ++lane_attempts;
         if (wants_defer(sc, R) && try_defer(df, R, fs, idx)) {
           active = false;
         } else {
-          step<float>(sc, R, scenes[f.scene].omega, reproject_now(sc, wave_steps),
-                      drift_now(wave_steps));
+          step<float>(sc, R, omega, reproject_now(sc, wave_steps), drift_now(wave_steps));
           if (R.term != kTermActive) {
+            const FrameScene& f = frame[fs];
             write_ray(R, f.out, idx, scenes[f.scene], map);
             active = false;
           }
@@ -350,10 +387,65 @@ __global__ void __launch_bounds__(256, kWaves) k_persistent(const SceneDev* __re
       if (drained && __ballot(active) == 0ull) break;
     }
   }
+  for (int off = 32; off > 0; off >>= 1) lane_attempts += __shfl_xor(lane_attempts, off);
   if (lane == 0) {
-    atomicAdd(&queue->attempts, attempts);
-    atomicAdd(&queue->slots, slots);
-    atomicAdd(&queue->refills, refills);
+    atomicAdd(&queue->attempts, (unsigned long long)lane_attempts);
+    atomicAdd(&queue->slots, 64ull * wave_steps);
+    atomicAdd(&queue->refills, (unsigned long long)refills);
+  }
+}
+
+// Whole-tile mode of the persistent kernel (refill threshold 64): each
+// wavefront takes one 8x8 tile (64 queue items, all of one scene) at a time
+// and traces it to completion before taking the next, so neighbouring pixels
+// stay together in a wave as in the plain dispatch, while the workgroups stay
+// resident and pull tiles of every scene from the one queue.
+template <bool kSingle, int kWaves>
+__global__ void __launch_bounds__(256, kWaves) k_persistent_tiles(
+    const SceneDev* __restrict__ scenes, const FrameScene* __restrict__ frame, uint32_t n_frame,
+    QueueDev* queue, MapView map, DeferBuf df, uint32_t /*interval*/, uint32_t /*threshold*/) {
+  const unsigned long long total = queue->total;
+  const uint32_t lane = __lane_id();
+  uint32_t lane_attempts = 0, tiles = 0, wave_steps = 0;
+  for (;;) {
+    unsigned long long base = 0;
+    if (lane == 0) base = atomicAdd(&queue->next, 64ull);
+    base = __shfl(base, 0);
+    if (base >= total) break;
+    ++tiles;
+    const unsigned long long item = base + lane;
+    const uint32_t fs = kSingle ? 0u : find_frame_scene(frame, n_frame, item);
+    const FrameScene& f = frame[fs];
+    const SceneDev& s = scenes[f.scene];
+    SceneT<float> sc = scene_f32(s);
+    hold_in_vgprs(sc);
+    uint32_t px, py;
+    image_pixel(uint32_t(item - f.first_item), s.tiles_x, px, py);
+    uint32_t it = 0;
+    if (px < s.width && py < s.height) {
+      const uint32_t idx = py * s.width + px;
+      RayT<float> R;
+      init_image_ray(s, sc, px, py, R);
+      bool deferred = false;
+      for (it = 1; R.term == kTermActive; ++it) {
+        if (wants_defer(sc, R) && try_defer(df, R, fs, idx)) {
+          deferred = true;
+          break;
+        }
+        step<float>(sc, R, s.omega, reproject_now(sc, it), drift_now(it));
+      }
+      lane_attempts += it - 1;
+      if (!deferred) write_ray(R, f.out, idx, s, map);
+    }
+    uint32_t longest = it;  // wave iterations of this tile: its longest lane
+    for (int off = 32; off > 0; off >>= 1) longest = max(longest, (uint32_t)__shfl_xor(longest, off));
+    wave_steps += longest > 0u ? longest - 1u : 0u;
+  }
+  for (int off = 32; off > 0; off >>= 1) lane_attempts += __shfl_xor(lane_attempts, off);
+  if (lane == 0) {
+    atomicAdd(&queue->attempts, (unsigned long long)lane_attempts);
+    atomicAdd(&queue->slots, 64ull * wave_steps);
+    atomicAdd(&queue->refills, (unsigned long long)tiles);
   }
 }
 
@@ -511,8 +603,20 @@ struct gl_trace_plan {
   gl_persistent_config pcfg = {256, 2, 8, 16, 2};
   bool timing = false;
   bool capturing = false;
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};  // main start/stop, finisher start/stop
-  bool finisher_timed = false;
+  // Timed launches: a ring of event quadruples (main start/stop, finisher
+  // start/stop); a slot is harvested into the sums before it is reused and
+  // when the durations are read, so every launch since the last read counts.
+  struct TimedLaunch {
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    bool used = false;
+    bool finisher = false;
+  };
+  static constexpr int kTimedRing = 64;
+  TimedLaunch timed[kTimedRing];
+  int timed_next = 0;
+  int timed_cur = -1;
+  double main_sum = 0.0, finish_sum = 0.0;
+  uint64_t main_n = 0, finish_n = 0;
 };
 
 struct gl_frame_graph {
@@ -586,8 +690,45 @@ gl_status ensure_defer(gl_trace_plan* p, size_t rays) {
   return GL_OK;
 }
 
+gl_status harvest(gl_trace_plan* p, gl_trace_plan::TimedLaunch& t) {
+  if (!t.used) return GL_OK;
+  t.used = false;
+  float ms = 0.0f;
+  GL_HIP(hipEventSynchronize(t.ev[1]));
+  GL_HIP(hipEventElapsedTime(&ms, t.ev[0], t.ev[1]));
+  p->main_sum += ms;
+  ++p->main_n;
+  if (t.finisher) {
+    GL_HIP(hipEventSynchronize(t.ev[3]));
+    GL_HIP(hipEventElapsedTime(&ms, t.ev[2], t.ev[3]));
+    p->finish_sum += ms;
+    ++p->finish_n;
+  }
+  return GL_OK;
+}
+
+// Event 0 opens a new timed launch (main kernel start), 1 closes the main
+// kernel, 2 and 3 bracket the f64 finisher of the same launch.
 void mark(gl_trace_plan* p, int which, hipStream_t st) {
-  if (p->timing && !p->capturing && p->ev[which]) (void)hipEventRecord(p->ev[which], st);
+  if (!p->timing || p->capturing) return;
+  if (which == 0) {
+    gl_trace_plan::TimedLaunch& t = p->timed[p->timed_next];
+    if (harvest(p, t) != GL_OK) {
+      p->timed_cur = -1;
+      return;
+    }
+    p->timed_cur = p->timed_next;
+    p->timed_next = (p->timed_next + 1) % gl_trace_plan::kTimedRing;
+    t.finisher = false;
+  }
+  if (p->timed_cur < 0) return;
+  gl_trace_plan::TimedLaunch& t = p->timed[p->timed_cur];
+  if (hipEventRecord(t.ev[which], st) != hipSuccess) {
+    p->timed_cur = -1;
+    return;
+  }
+  if (which == 1) t.used = true;
+  if (which == 3) t.finisher = true;
 }
 
 MapView map_view(const gl_trace_plan* p) {
@@ -626,7 +767,6 @@ gl_status launch_finisher(gl_trace_plan* p, hipStream_t st) {
                      p->d_scenes, p->d_frame, map_view(p), defer_buf(p));
   GL_HIP(hipGetLastError());
   mark(p, 3, st);
-  p->finisher_timed = true;
   return GL_OK;
 }
 
@@ -664,7 +804,19 @@ gl_status build_frame_table(gl_trace_plan* p, const uint32_t* scenes, size_t n_s
   return GL_OK;
 }
 
-const void* persistent_kernel(bool single, uint32_t waves) {
+// Lane-refill kernels for waves_per_simd 2..4; whole-tile kernels (refill
+// threshold 64) for 2 and 3.
+const void* persistent_kernel(bool single, uint32_t waves, bool tiles) {
+  if (tiles) {
+    switch (waves) {
+      case 2:
+        return single ? (const void*)k_persistent_tiles<true, 2> : (const void*)k_persistent_tiles<false, 2>;
+      case 3:
+        return single ? (const void*)k_persistent_tiles<true, 3> : (const void*)k_persistent_tiles<false, 3>;
+      default:
+        return nullptr;
+    }
+  }
   switch (waves) {
     case 2:
       return single ? (const void*)k_persistent<true, 2> : (const void*)k_persistent<false, 2>;
@@ -694,15 +846,18 @@ gl_status launch_persistent(gl_trace_plan* p, const FrameScene* d_table, size_t 
   // Every workgroup must be resident at once: clamp the requested blocks
   // per CU to what the kernel's register budget allows.
   int resident = 0;
-  const void* kern = persistent_kernel(n_table == 1, c.waves_per_simd);
-  if (!kern) return fail(GL_ERR_INVALID_ARGUMENT, "waves_per_simd must be 2, 3 or 4");
+  const bool tiles = c.refill_threshold >= 64u;
+  const void* kern = persistent_kernel(n_table == 1, c.waves_per_simd, tiles);
+  if (!kern)
+    return fail(GL_ERR_INVALID_ARGUMENT,
+                tiles ? "whole-tile mode (refill_threshold 64) takes waves_per_simd 2 or 3"
+                      : "waves_per_simd must be 2, 3 or 4");
   GL_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, kern, int(c.block_size), 0));
   int per_cu = std::max(1, std::min(int(c.blocks_per_cu), resident));
   dim3 grid(std::max(1, p->cu_count * per_cu));
   dim3 block(c.block_size);
   DeferBuf df = defer_buf(p);
   if (!f64) df.rec = nullptr;
-  p->finisher_timed = false;
   mark(p, 0, st);
   {
     const SceneDev* a0 = p->d_scenes;
@@ -723,7 +878,6 @@ gl_status launch_persistent(gl_trace_plan* p, const FrameScene* d_table, size_t 
                        p->d_scenes, d_table, map_view(p), df);
     GL_HIP(hipGetLastError());
     mark(p, 3, st);
-    p->finisher_timed = true;
   }
   return GL_OK;
 }
@@ -813,8 +967,9 @@ gl_status gl_trace_plan_destroy(gl_trace_plan* p) {
   if (p->d_defer) (void)hipFree(p->d_defer);
   if (p->d_defer_count) (void)hipFree(p->d_defer_count);
   if (p->frame_copied) (void)hipEventDestroy(p->frame_copied);
-  for (hipEvent_t e : p->ev)
-    if (e) (void)hipEventDestroy(e);
+  for (auto& t : p->timed)
+    for (hipEvent_t e : t.ev)
+      if (e) (void)hipEventDestroy(e);
   delete p;
   return GL_OK;
 }
@@ -850,7 +1005,6 @@ gl_status gl_trace_encode(gl_trace_plan* p, const gl_tile_region* region,
   DeferBuf df = defer_buf(p);
   if (!f64) df.rec = nullptr;
   size_t blocks = size_t(region->tiles_x) * region->tiles_y;
-  p->finisher_timed = false;
   mark(p, 0, st);
   hipLaunchKernelGGL(k_plain_image, dim3(uint32_t(blocks)), dim3(64), 0, st, p->d_scenes,
                      p->d_frame, region->tile_x, region->tile_y, region->tiles_x, map_view(p), df);
@@ -909,7 +1063,6 @@ gl_status gl_geodesic_batch_trace(gl_trace_plan* p, uint32_t scene, const double
   hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, st, p->d_queue, 0ull, p->d_defer_count);
   DeferBuf df = defer_buf(p);
   if (!f64) df.rec = nullptr;
-  p->finisher_timed = false;
   mark(p, 0, st);
   hipLaunchKernelGGL(k_geodesic_batch, dim3(uint32_t((n + 63) / 64)), dim3(64), 0, st,
                      p->d_scenes, p->d_frame, states8, directions, uint32_t(n), map_view(p), df);
@@ -934,26 +1087,31 @@ gl_status gl_trace_plan_queue_stats(gl_trace_plan* p, gl_queue_stats* stats, voi
 gl_status gl_trace_plan_enable_timing(gl_trace_plan* p, int enable) {
   if (!p) return fail(GL_ERR_INVALID_ARGUMENT, "plan is required");
   if (enable) {
-    for (hipEvent_t& e : p->ev)
-      if (!e) GL_HIP(hipEventCreate(&e));
+    for (auto& t : p->timed)
+      for (hipEvent_t& e : t.ev)
+        if (!e) GL_HIP(hipEventCreate(&e));
   }
+  for (auto& t : p->timed) t.used = false;
+  p->timed_cur = -1;
+  p->main_sum = p->finish_sum = 0.0;
+  p->main_n = p->finish_n = 0;
   p->timing = enable != 0;
   return GL_OK;
 }
 
-gl_status gl_trace_plan_kernel_ms(gl_trace_plan* p, double* main_ms, double* finish_ms) {
+gl_status gl_trace_plan_kernel_ms(gl_trace_plan* p, double* main_ms, double* finish_ms,
+                                  uint64_t* launches) {
   if (!p || !main_ms || !finish_ms) return fail(GL_ERR_INVALID_ARGUMENT, "plan and outputs are required");
   if (!p->timing) return fail(GL_ERR_INVALID_ARGUMENT, "timing is not enabled");
-  float ms = 0.0f;
-  GL_HIP(hipEventSynchronize(p->ev[1]));
-  GL_HIP(hipEventElapsedTime(&ms, p->ev[0], p->ev[1]));
-  *main_ms = ms;
-  *finish_ms = 0.0;
-  if (p->finisher_timed) {
-    GL_HIP(hipEventSynchronize(p->ev[3]));
-    GL_HIP(hipEventElapsedTime(&ms, p->ev[2], p->ev[3]));
-    *finish_ms = ms;
+  for (int i = 0; i < gl_trace_plan::kTimedRing; ++i) {
+    gl_status s = harvest(p, p->timed[(p->timed_next + i) % gl_trace_plan::kTimedRing]);
+    if (s) return s;
   }
+  *main_ms = p->main_n ? p->main_sum / double(p->main_n) : 0.0;
+  *finish_ms = p->finish_n ? p->finish_sum / double(p->finish_n) : 0.0;
+  if (launches) *launches = p->main_n;
+  p->main_sum = p->finish_sum = 0.0;
+  p->main_n = p->finish_n = 0;
   return GL_OK;
 }
 

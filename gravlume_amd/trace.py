@@ -186,11 +186,13 @@ class TracePipeline:
         """HIP events around the main kernel and the f64 finisher of each launch."""
         _capi.check(self._lib.gl_trace_plan_enable_timing(self._handle, int(enable)))
 
-    def kernel_ms(self) -> tuple[float, float]:
-        """(main kernel ms, f64 finisher ms) of the last launch."""
-        a, b = C.c_double(), C.c_double()
-        _capi.check(self._lib.gl_trace_plan_kernel_ms(self._handle, C.byref(a), C.byref(b)))
-        return a.value, b.value
+    def kernel_ms(self) -> tuple[float, float, int]:
+        """(main kernel ms, f64 finisher ms, launches): mean durations over every
+        launch since timing was enabled or this was last called."""
+        a, b, n = C.c_double(), C.c_double(), C.c_uint64()
+        _capi.check(self._lib.gl_trace_plan_kernel_ms(self._handle, C.byref(a), C.byref(b),
+                                                      C.byref(n)))
+        return a.value, b.value, int(n.value)
 
     def frame_graph(self, targets, scenes=None, stream=None) -> "FrameGraph":
         return FrameGraph(self, targets, scenes, stream)

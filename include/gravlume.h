@@ -166,11 +166,14 @@ gl_status gl_trace_encode_persistent(gl_trace_plan* plan, const uint32_t* scenes
 
 /* Launch parameters of the persistent kernel (set by the autotuner). */
 typedef struct gl_persistent_config {
-  uint32_t block_size;       /* lanes per workgroup: 64..1024, multiple of 64 */
-  uint32_t blocks_per_cu;    /* resident workgroups per CU */
+  uint32_t block_size;       /* lanes per workgroup: 64, 128, 192 or 256 */
+  uint32_t blocks_per_cu;    /* resident workgroups per CU (capped by occupancy) */
   uint32_t refill_interval;  /* DP5 attempts between refill checks */
-  uint32_t refill_threshold; /* free lanes that trigger a refill */
-  uint32_t waves_per_simd;   /* register budget: 2, 3 or 4 resident waves per SIMD */
+  uint32_t refill_threshold; /* free lanes that trigger a lane refill, 1..63;
+                                64 = whole-tile mode: a wavefront takes one
+                                8x8 tile at a time and finishes it first */
+  uint32_t waves_per_simd;   /* register budget: 2, 3 or 4 resident waves per
+                                SIMD (2 or 3 in whole-tile mode) */
 } gl_persistent_config;
 gl_status gl_trace_plan_set_persistent_config(gl_trace_plan* plan,
                                               const gl_persistent_config* config);
@@ -197,9 +200,13 @@ gl_status gl_trace_plan_queue_stats(gl_trace_plan* plan, gl_queue_stats* stats, 
 /* Kernel timing: when enabled, HIP events on the launch stream bracket the
  * main trace kernel and the f64 near-horizon finisher of every subsequent
  * encode call (not inside graph capture). gl_trace_plan_kernel_ms
- * synchronizes on those events and returns the last launch's durations. */
+ * synchronizes on those events and returns the mean durations over every
+ * launch since timing was enabled or the durations were last read (then
+ * starts a new count); `launches` (may be NULL) receives how many main
+ * kernels the mean covers. Enabling timing clears the count. */
 gl_status gl_trace_plan_enable_timing(gl_trace_plan* plan, int enable);
-gl_status gl_trace_plan_kernel_ms(gl_trace_plan* plan, double* main_ms, double* finish_ms);
+gl_status gl_trace_plan_kernel_ms(gl_trace_plan* plan, double* main_ms, double* finish_ms,
+                                  uint64_t* launches);
 
 /* Frame capture: records the persistent frame (queue reset + launch) into a
  * hipGraph once; gl_frame_graph_launch replays it. */

@@ -124,11 +124,16 @@ def test_kerr_1080p_frame_sample_matches_oracle(gl, oracle):
     _compare(h, A, idx)
 
 
-def test_persistent_matches_plain(gl):
+@pytest.mark.parametrize("mode", [dict(refill_threshold=16, refill_interval=8, waves_per_simd=2),
+                                  dict(refill_threshold=64, refill_interval=8, waves_per_simd=2),
+                                  dict(refill_threshold=64, refill_interval=8, waves_per_simd=3)],
+                         ids=["lane-refill", "whole-tile", "whole-tile-3-waves"])
+def test_persistent_matches_plain(gl, mode):
     from gravlume_amd import scene as S
     import torch
     sc = S.single_frame_1080p(width=480, height=270)
     pipe = gl.TracePipeline([sc])
+    pipe.set_persistent_config(block_size=256, blocks_per_cu=mode["waves_per_simd"], **mode)
     a, b = pipe.create_target(0), pipe.create_target(0)
     pipe.encode_batch(a)
     pipe.encode_persistent([b])
@@ -168,6 +173,7 @@ def test_multi_scene_persistent_matches_single_scene_launches(gl):
     scenes = S.uneven_scenes(count=6, seed=5)
     for s in scenes:
         s.width = s.height = 64
+    scenes[1].width = 67  # a ragged viewport among them
     pipe = gl.TracePipeline(scenes)
     multi = [pipe.create_target(i) for i in range(len(scenes))]
     pipe.encode_persistent(multi)
@@ -176,11 +182,17 @@ def test_multi_scene_persistent_matches_single_scene_launches(gl):
         t = pipe.create_target(i)
         pipe.encode_persistent([t], scenes=[i])
         single.append(t)
+    pipe.set_persistent_config(block_size=256, blocks_per_cu=2, refill_interval=8,
+                               refill_threshold=64, waves_per_simd=2)
+    tiled = [pipe.create_target(i) for i in range(len(scenes))]
+    pipe.encode_persistent(tiled)
     torch.cuda.synchronize()
-    for m, s in zip(multi, single):
-        hm, hs = m.to_host(), s.to_host()
+    for m, s, t in zip(multi, single, tiled):
+        hm, hs, ht = m.to_host(), s.to_host(), t.to_host()
         assert np.all(hm["termination"] > 0)
+        assert np.all(ht["termination"] > 0)
         assert np.mean(hm["termination"] == hs["termination"]) >= 0.999
+        assert np.mean(ht["termination"] == hs["termination"]) >= 0.999
 
 
 def test_geodesic_batch_reproduces_the_b6_fixture(gl):
