@@ -179,6 +179,20 @@ def main():
 
     t_pers, k_pers, f_pers = timed(lambda: pipe.encode_persistent([target]))
     stats = pipe.queue_stats()
+    # host-inclusive rate: each frame's 60 B/ray of output planes copied to
+    # pinned host memory after the trace (PCIe), frames back to back
+    planes = ("status", "steps", "rejected", "observable", "travel_time", "affine", "drift",
+              "branch", "radiance")
+    pinned = {k: torch.empty(getattr(target, k).shape, dtype=getattr(target, k).dtype,
+                             pin_memory=True) for k in planes}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        pipe.encode_persistent([target])
+        for k in planes:
+            pinned[k].copy_(getattr(target, k), non_blocking=True)
+    torch.cuda.synchronize()
+    t_pcie = time.perf_counter() - t0
     t_plain, k_plain, f_plain = timed(lambda: pipe.encode_batch(plain_target))
     host = target.to_host()
     plain_host = plain_target.to_host()
@@ -209,6 +223,7 @@ def main():
                                "escape r=1000M into a 2048x1024 seeded RGB map",
                    "rays_per_step": rays, "kernel": "k_persistent"},
         "plain_rays_per_s": plain_value,
+        "host_inclusive_rays_per_s": rays * args.steps / t_pcie,
         "persistent_over_plain": value / plain_value,
         "kernel_ms": {"persistent": k_pers, "persistent_f64_finisher": f_pers, "plain": k_plain,
                       "plain_f64_finisher": f_plain},

@@ -18,6 +18,8 @@
 // path, double in the near-horizon finisher.
 #pragma once
 
+#include <cmath>
+
 #include "gl_types.h"
 #include "ks_math.h"
 

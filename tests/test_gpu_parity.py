@@ -122,6 +122,34 @@ def test_kerr_1080p_frame_sample_matches_oracle(gl, oracle):
     idx = np.random.default_rng(11).choice(sc.rays, 768, replace=False)
     A = _oracle_outcomes(oracle, sc, idx)
     _compare(h, A, idx)
+    _check_radiance(h, sc, pipe.escape_map_rgb)
+
+
+def _check_radiance(h, sc, rgb):
+    """Radiance planes from the observables: the equirectangular texel of the
+    escape direction, and (g T_in (r/r_in)^-3/4, g^4 (r/6)^-3, 0) on the disk."""
+    esc = h["termination"] == 2
+    d = h["observable"][esc].astype(np.float64)
+    _, H, W = rgb.shape
+    lon = np.arctan2(d[:, 1], d[:, 0])
+    colat = np.arccos(np.clip(d[:, 2], -1.0, 1.0))
+    fu, fv = (lon + np.pi) * W / (2 * np.pi), colat * H / np.pi
+    iu = np.clip(fu.astype(np.int64), 0, W - 1)
+    iv = np.clip(fv.astype(np.int64), 0, H - 1)
+    # texel edges within f32 rounding of the direction may round either way
+    clear = (np.abs(fu - np.round(fu)) > 1e-3) & (np.abs(fv - np.round(fv)) > 1e-3)
+    want = rgb[:, iv, iu].T
+    got = h["radiance"][esc]
+    assert np.mean(np.all(got[clear] == want[clear], axis=1)) >= 0.9999
+    srf = h["termination"] == 7
+    o = h["observable"][srf].astype(np.float64)
+    r_in = sc.disk_radii()[0]
+    t = o[:, 2] * sc.temperature_inner * (o[:, 0] / r_in) ** -0.75
+    b = o[:, 2] ** 4 * (o[:, 0] / 6.0) ** -3
+    got = h["radiance"][srf].astype(np.float64)
+    assert np.max(np.abs(got[:, 0] / t - 1)) <= 1e-5
+    assert np.max(np.abs(got[:, 1] / b - 1)) <= 1e-5
+    assert np.all(got[:, 2] == 0)
 
 
 @pytest.mark.parametrize("mode", [dict(refill_threshold=16, refill_interval=8, waves_per_simd=2),

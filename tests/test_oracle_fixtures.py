@@ -166,3 +166,46 @@ def test_kerr_surface_observable_fixture(oracle, policy):
     assert abs(o.travel_time - float(e["travel_time"])) <= float(t["travel_time_abs"])
     assert abs(o.emitted_intensity - float(e["emitted_bolometric_intensity"])) <= float(t["bolometric_intensity_abs"])
     assert abs(o.observed_intensity - float(e["observed_bolometric_intensity"])) <= float(t["bolometric_intensity_abs"])
+
+
+@pytest.mark.parametrize("name", ["schwarzschild_scatter_b6", "schwarzschild_scatter_near_critical",
+                                  "schwarzschild_capture_near_critical"])
+def test_winding_follows_the_fixture_azimuth_advance(oracle, name):
+    """Signed winding (geodesic.rs:406-418: the azimuth jumps across +-pi) of an
+    equatorial Schwarzschild orbit that starts at azimuth 0 and turns
+    counter-clockwise through the fixture's azimuth advance: the orbit crosses
+    the -x half-axis floor((dphi + pi) / 2 pi) times."""
+    f, _, _, out = _geodesic(oracle, name, "strict")
+    dphi = float(f["expected"]["azimuth_advance"])
+    assert out.azimuth_winding == math.floor((dphi + math.pi) / (2 * math.pi))
+    end = math.atan2(out.state[2], out.state[1])
+    assert abs(end + 2 * math.pi * out.azimuth_winding - out.azimuth_advance) <= 1e-9
+
+
+@pytest.mark.parametrize("name", ["schwarzschild_scatter_b6", "schwarzschild_scatter_near_critical"])
+@pytest.mark.parametrize("incl", [0.35, 1.1])
+def test_equatorial_crossings_follow_the_fixture_azimuth_advance(oracle, name, incl):
+    """Rotate the fixture orbit into a plane inclined by `incl` about the x axis
+    and start it 0.3 rad past the ascending node (a rotation is a symmetry of
+    Schwarzschild, so the in-plane angle still advances by the fixture's
+    azimuth advance). The orbit then crosses the equator at in-plane angles
+    pi, 2 pi, ...: floor((0.3 + dphi) / pi) committed crossings
+    (record_equatorial_crossing, geodesic.rs:362-383); the radial turning
+    count stays 1."""
+    f = FIX[name]
+    st = oracle.spacetime(1.0, 0.0, 0.0, 1.0)
+    x = np.array([float(v) for v in f["position_txyz"]][1:])
+    p = np.array([float(v) for v in f["momentum_covariant"]][1:])
+    psi0 = 0.3
+    c, s = math.cos(psi0), math.sin(psi0)
+    rz = np.array([[c, -s, 0], [s, c, 0], [0, 0, 1]])          # start past the node
+    rx = np.array([[1, 0, 0], [0, math.cos(incl), -math.sin(incl)],
+                   [0, math.sin(incl), math.cos(incl)]])        # tilt the orbit plane
+    R = rx @ rz
+    y = [float(f["position_txyz"][0]), *(R @ x), float(f["momentum_covariant"][0]), *(R @ p)]
+    out = oracle.trace(st, oracle.policy("strict"), oracle.events(float(f["escape_radius"])), y,
+                       float(f["affine_direction"]))
+    dphi = float(f["expected"]["azimuth_advance"])
+    assert out.termination == oracle.TERM_ESCAPE
+    assert out.equatorial_crossings == math.floor((psi0 + dphi) / math.pi)
+    assert out.radial_turnings == 1
