@@ -35,6 +35,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-vjp", action="store_true", help="skip the adjoint leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--accuracy-rays", type=int, default=2048)
     ap.add_argument("--persistent-config", type=str, default=None,
@@ -124,6 +125,54 @@ def cpu_baseline(scene, gpu_host, seconds, accuracy_rays, seed=20261203):
                    source_r_abs_max=float(np.max(np.abs(g[:, 0] - A["source_radius"][srf]))))
     err["null_drift_p99"] = float(np.quantile(gpu_host["drift"][idx][:, 0], 0.99))
     return base, err
+
+
+def vjp_leg(g, S, torch, repeats=5):
+    """Adjoint workload: 512x512, a = 0.9, Q = 0.2, inclination 70 deg; loss
+    sum (g - g*)^2 over surface pixels with g* traced at (0.92, 0.18, 71 deg);
+    checkpoints every 32 accepted steps. Kernel times from the library's
+    events: the plain forward frame (trace + f64 finisher) against the
+    adjoint's recording forward and its backward."""
+    sc = S.adjoint_scene()
+    pipe = g.TracePipeline([sc])
+    pipe.enable_timing(True)
+    tgt = pipe.create_target(0)
+    pipe.encode_batch(tgt)
+    torch.cuda.synchronize()
+    pipe.kernel_ms()
+    for _ in range(repeats):
+        pipe.encode_batch(tgt)
+    torch.cuda.synchronize()
+    main_ms, fin_ms, _ = pipe.kernel_ms()
+    tsc = S.adjoint_scene(spin=0.92, charge=0.18, inclination_deg=71.0)
+    tpipe = g.TracePipeline([tsc])
+    ttg = tpipe.create_target(0)
+    tpipe.encode_batch(ttg)
+    torch.cuda.synchronize()
+    n = ttg.rays
+    gstar = torch.where((ttg.status & 15) == 7, ttg.observable[2 * n:], torch.zeros_like(ttg.affine))
+    fwd, bwd, loss, grads = [], [], 0.0, None
+    for _ in range(repeats + 1):
+        v = g.VjpTrace(pipe, 0, checkpoint_interval=32)
+        torch.cuda.synchronize()
+        f_ms = pipe.kernel_ms()[0]
+        gg = v.g
+        mask = (gg > 0) & (gstar > 0)
+        resid = torch.where(mask, gg - gstar, torch.zeros_like(gg))
+        loss = float((resid * resid).sum())
+        grads = v.backward(2.0 * resid)
+        b_ms = pipe.kernel_ms()[0]
+        v.close()
+        fwd.append(f_ms)
+        bwd.append(b_ms)
+    fwd, bwd = float(np.median(fwd[1:])), float(np.median(bwd[1:]))
+    plain = main_ms + fin_ms
+    return {"workload": "512x512 Kerr-Newman a=0.9 Q=0.2 incl 70deg r_obs 100M FOV 20deg, disk "
+                        "[r_ISCO, 25M], f32 rtol 1e-6, K=32; loss sum (g-g*)^2, g* at "
+                        "(0.92, 0.18, 71deg)",
+            "plain_forward_ms": plain, "vjp_forward_ms": fwd, "vjp_backward_ms": bwd,
+            "cost_ratio": (fwd + bwd) / plain, "backward_over_forward": bwd / fwd,
+            "loss": loss, "grad_a_q_incl": [float(x) for x in grads]}
 
 
 def main():
@@ -241,6 +290,8 @@ def main():
                      "flops_per_rhs": rhs_flops, "attempts_per_launch": attempts},
         "cpu_baseline": None,
     }
+    if not args.no_vjp:
+        line["vjp"] = vjp_leg(g, S, torch)
     if not args.no_cpu:
         base, err = cpu_baseline(scene, host, args.cpu_seconds, args.accuracy_rays)
         line["cpu_baseline"] = base

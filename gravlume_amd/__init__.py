@@ -11,9 +11,10 @@ from ._capi import GravlumeError
 _capi.load()
 
 from .scene import Scene, escape_map, isco_prograde, outer_horizon  # noqa: E402
-from .trace import (FrameGraph, GeodesicBatch, GpuTraceInputError, TileRegion,  # noqa: E402
-                    TracePipeline, TraceTarget, flop_constants)
+from .trace import (FrameGraph, FrequencyRatioImage, GeodesicBatch,  # noqa: E402
+                    GpuTraceInputError, TileRegion, TracePipeline, TraceTarget, VjpTrace,
+                    flop_constants)
 
-__all__ = ["FrameGraph", "GeodesicBatch", "GpuTraceInputError", "GravlumeError", "Scene",
+__all__ = ["FrameGraph", "FrequencyRatioImage", "GeodesicBatch", "VjpTrace", "GpuTraceInputError", "GravlumeError", "Scene",
            "TileRegion", "TracePipeline", "TraceTarget", "escape_map", "flop_constants",
            "isco_prograde", "outer_horizon"]

@@ -1,0 +1,72 @@
+// Host state of a trace plan, shared by the trace and adjoint entry points.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/gravlume.h"
+#include "gl_device.h"
+#include "gl_scene.h"
+#include "gl_types.h"
+
+namespace gl {
+// Records `msg` as the calling thread's last error and returns `st`.
+gl_status fail(gl_status st, const std::string& msg);
+}  // namespace gl
+
+#define GL_HIP(call)                                                                       \
+  do {                                                                                     \
+    hipError_t e_ = (call);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      return ::gl::fail(GL_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+struct gl_trace_plan {
+  using SceneDev = gl::SceneDev;
+  using SceneHost = gl::SceneHost;
+  using QueueDev = gl::QueueDev;
+  using FrameScene = gl::FrameScene;
+  using DeferRecord = gl::DeferRecord;
+  std::vector<SceneDev> scenes;
+  std::vector<SceneHost> host;
+  SceneDev* d_scenes = nullptr;
+  float* d_map = nullptr;
+  uint32_t map_w = 0, map_h = 0;
+  QueueDev* d_queue = nullptr;
+  FrameScene* d_frame = nullptr;
+  FrameScene* h_frame = nullptr;  // pinned staging of the frame table
+  size_t frame_capacity = 0;
+  std::vector<FrameScene> frame_shadow;  // what d_frame holds (or will hold)
+  hipEvent_t frame_copied = nullptr;     // completion of the last staging copy
+  bool frame_copy_pending = false;
+  DeferRecord* d_defer = nullptr;
+  unsigned int* d_defer_count = nullptr;
+  uint32_t defer_capacity = 0;
+  int cu_count = 256;
+  gl_persistent_config pcfg = {256, 2, 8, 16, 2};
+  bool timing = false;
+  bool capturing = false;
+  // Timed launches: a ring of event quadruples (main start/stop, finisher
+  // start/stop); a slot is harvested into the sums before it is reused and
+  // when the durations are read, so every launch since the last read counts.
+  struct TimedLaunch {
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    bool used = false;
+    bool finisher = false;
+  };
+  static constexpr int kTimedRing = 64;
+  TimedLaunch timed[kTimedRing];
+  int timed_next = 0;
+  int timed_cur = -1;
+  double main_sum = 0.0, finish_sum = 0.0;
+  uint64_t main_n = 0, finish_n = 0;
+};
+
+
+namespace gl {
+// Records timing event `which` (0/1 main kernel start/stop, 2/3 finisher)
+// of the current launch on `st` when the plan's timing is enabled.
+void plan_mark(gl_trace_plan* p, int which, hipStream_t st);
+}  // namespace gl

@@ -1,20 +1,442 @@
-// Discrete-adjoint VJP entry points (placeholder until the adjoint kernels land).
+// Discrete adjoint of the f32 trace: vector-Jacobian products of per-pixel
+// frequency ratios g of surface hits with respect to the spin a, the charge
+// Q, the observer inclination and each pixel's sight-plane direction.
+//
+// Forward (k_vjp_forward): the plain image trace, plus a record of every
+// accepted step size, the state every K accepted steps (checkpoints), the
+// state before the terminal step, and E.
+//
+// Backward (k_vjp_backward), one ray per lane:
+//   1. the terminal step: the surface observable's gradient (forward-mode
+//      duals), the implicit dependence of the crossing fraction theta* on the
+//      step (z(theta*) = 0), and the reverse sweep through the quartic dense
+//      output and its seven stages;
+//   2. segment by segment from the last checkpoint down: the segment's states
+//      are recomputed with the frozen step sizes into a per-lane buffer, then
+//      each step is reversed stage by stage (rhs_vjp, adjoint.h);
+//   3. the initial state: its tangents with respect to (a, Q, inclination)
+//      come from f64 central differences of the host tetrad construction,
+//      combined on the device with forward-mode duals of the pixel's sight
+//      momentum; the direction gradient uses the same duals.
+// Step sizes are frozen (the derivative of the discrete map at fixed h), and
+// the Kahan compensation and null re-projection of the forward pass are
+// treated as identity maps.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cmath>
+#include <string>
+
 #include "../../include/gravlume.h"
+#include "adjoint.h"
+#include "gl_device.h"
+#include "gl_plan.h"
+#include "gl_scene.h"
+#include "gl_types.h"
+#include "trace_device.h"
+
+namespace gl {
+
+// Tangents of the observer state with respect to (a, Q, inclination).
+struct VjpTangents {
+  float obs[3][3];     // d(x, y, z)_obs / d param
+  float ut[3];         // d u^t / d param
+  float right[3][4];   // d e_R / d param
+  float up[3][4];      // d e_U / d param
+  float arrival[3][4]; // d e_A / d param
+  float dq2_dq;        // d Q^2 / d Q
+};
+
+struct VjpBufs {
+  float* h;       // [cap][rays] accepted step sizes
+  float* ckpt;    // [ncp][6][rays] state before steps 0, K, 2K, ...
+  float* term;    // [6][rays] state before the terminal step
+  float* E;       // [rays]
+  uint32_t* n;    // [rays] accepted steps incl. the terminal one; 0 = no gradient
+  float* seg;     // [K][6][rays] backward segment states
+  uint32_t rays, cap, K, ncp;
+};
+
+__device__ __forceinline__ bool vjp_pixel(const SceneDev& s, uint32_t& px, uint32_t& py) {
+  px = (blockIdx.x % s.tiles_x) * 8u + (threadIdx.x & 7u);
+  py = (blockIdx.x / s.tiles_x) * 8u + (threadIdx.x >> 3);
+  return px < s.width && py < s.height;
+}
+
+__global__ void __launch_bounds__(64) k_vjp_forward(const SceneDev* __restrict__ scenes,
+                                                    uint32_t scene, OutPlanes out, MapView map,
+                                                    VjpBufs vb) {
+  const SceneDev& s = scenes[scene];
+  SceneT<float> sc = scene_f32(s);
+  hold_in_vgprs(sc);
+  uint32_t px, py;
+  if (!vjp_pixel(s, px, py)) return;
+  const uint32_t idx = py * s.width + px;
+  const size_t rays = vb.rays;
+  RayT<float> R;
+  init_image_ray(s, sc, px, py, R);
+  bool overflow = false;
+  for (uint32_t it = 1; R.term == kTermActive; ++it) {
+    const float pre[6] = {R.x - R.cx, R.y - R.cy, R.z - R.cz, R.px, R.py, R.pz};
+    const float h = R.h;
+    const uint32_t acc = R.accepted;
+    step<float>(sc, R, s.omega, reproject_now(sc, it), drift_now(it));
+    if (R.accepted != acc) {
+      if (acc < vb.cap) {
+        vb.h[size_t(acc) * rays + idx] = h;
+        if (acc % vb.K == 0u) {
+#pragma unroll
+          for (int c = 0; c < 6; ++c) vb.ckpt[(size_t(acc / vb.K) * 6 + c) * rays + idx] = pre[c];
+        }
+      } else {
+        overflow = true;
+      }
+      if (R.term != kTermActive) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) vb.term[size_t(c) * rays + idx] = pre[c];
+      }
+    }
+  }
+  vb.E[idx] = R.E;
+  vb.n[idx] = (R.term == kTermSurface && !overflow) ? R.accepted : 0u;
+  write_ray(R, out, idx, s, map);
+}
+
+// Adjoint of the terminal (surface-crossing) step, seeded by gbar = dL/dg.
+// Returns false when the observable is not differentiable there.
+__device__ __forceinline__ bool terminal_adjoint(const SceneT<float>& sc, const SceneDev& s,
+                                                 const float y[6], float E, float h, float gbar,
+                                                 float lam[6], RhsBar<float>& acc) {
+  float K[7][6], Kt[7][7];
+  const float th = terminal_stages<float>(sc, y, E, h, K, Kt);
+  float yT[6], dtl;
+  dense_state<float>(Kt, h, y, th, yT, dtl);
+  // observable g(r, E, L_z, a, Q^2) by forward-mode duals
+  Geom<float> gT = geometry<float>(yT[0], yT[1], yT[2], sc.mass, sc.spin, sc.spin2, sc.charge2,
+                                   sc.chart);
+  const float lz = yT[0] * yT[4] - yT[1] * yT[3];
+  Dual5 o[3];
+  if (!surface_observable<Dual5>(Dual5(1.0), Dual5::var(sc.spin, 3), Dual5::var(sc.charge2, 4),
+                                 Dual5(double(sc.chart)), Dual5(double(s.omega)),
+                                 Dual5(double(yT[0])), Dual5(double(yT[1])), Dual5::var(gT.r, 0),
+                                 Dual5::var(E, 1), Dual5::var(lz, 2), o))
+    return false;
+  const float gr = o[2].d[0], gE = o[2].d[1], glz = o[2].d[2];
+  const float rr = gT.r;
+  const float rho2 = yT[0] * yT[0] + yT[1] * yT[1] + yT[2] * yT[2];
+  const float dr_da2 = -(rr * rr - yT[2] * yT[2]) /
+                       (4.0f * rr * rr * rr - 2.0f * rr * (rho2 - sc.spin2));
+  const float lT[6] = {gbar * (gr * gT.grx + glz * yT[4]), gbar * (gr * gT.gry - glz * yT[3]),
+                       gbar * gr * gT.grz, -gbar * glz * yT[1], gbar * glz * yT[0], 0.0f};
+  acc.E += gbar * gE;
+  acc.spin += gbar * o[2].d[3];
+  acc.spin2 += gbar * gr * dr_da2;
+  acc.charge2 += gbar * o[2].d[4];
+  return terminal_reverse<float>(sc, y, E, h, K, th, lT, lam, acc);
+}
+
+struct Grad5 {
+  float a, q, incl, sx, sy;
+};
+
+// The sight momentum of pixel (px, py) in forward-mode duals over
+// (a, Q, inclination, s_x, s_y), contracted with the initial-state adjoint
+// (lam, lamE). Out of line: the duals' registers stay out of the trace loop.
+__device__ __noinline__ Grad5 initial_contract(const SceneDev& s, const VjpTangents* tg,
+                                               uint32_t px, uint32_t py, float l0, float l1,
+                                               float l2, float l3, float l4, float l5,
+                                               float lE) {
+  const float nx = 2.0f * (float(px) + s.sub_x) / float(s.width) - 1.0f;
+  const float ny = 1.0f - 2.0f * (float(py) + s.sub_y) / float(s.height);
+  const Dual5 sx = Dual5::var(s.aspect * s.tan_half_fov * nx, 3);
+  const Dual5 sy = Dual5::var(s.tan_half_fov * ny, 4);
+  auto tang = [](float v, const float t[3][4], int c) {
+    Dual5 r(0.0);
+    r.v = v;
+    for (int k = 0; k < 3; ++k) r.d[k] = t[k][c];
+    return r;
+  };
+  const Dual5 inv = Dual5(1.0) / gl_sqrt<Dual5>(Dual5(1.0) + sx * sx + sy * sy);
+  Dual5 d[4];
+  for (int i = 0; i < 4; ++i)
+    d[i] = (sx * tang(s.right[i], tg->right, i) + sy * tang(s.up[i], tg->up, i) -
+            tang(s.arrival[i], tg->arrival, i)) *
+           inv;
+  Dual5 ut(0.0);
+  ut.v = s.u_t;
+  for (int k = 0; k < 3; ++k) ut.d[k] = tg->ut[k];
+  const Dual5 om(double(s.omega));
+  const Dual5 pu[4] = {om * (ut - d[0]), -(om * d[1]), -(om * d[2]), -(om * d[3])};
+  Dual5 ob[3];
+  const float ov[3] = {s.obs_x, s.obs_y, s.obs_z};
+  for (int c = 0; c < 3; ++c) {
+    ob[c] = Dual5(0.0);
+    ob[c].v = ov[c];
+    for (int k = 0; k < 3; ++k) ob[c].d[k] = tg->obs[k][c];
+  }
+  const Dual5 spin = Dual5::var(s.spin, 0);
+  Dual5 q2(0.0);
+  q2.v = s.charge2;
+  q2.d[1] = tg->dq2_dq;
+  Geom<Dual5> g = geometry<Dual5>(ob[0], ob[1], ob[2], Dual5(double(s.mass)), spin, spin * spin,
+                                  q2, Dual5(double(s.chart)));
+  const Dual5 lp = pu[0] + g.lx * pu[1] + g.ly * pu[2] + g.lz * pu[3];
+  const Dual5 fl = g.f * lp;
+  const Dual5 E = pu[0] - fl;  // E = -p_t, p_t = -pu^t + f (l.pu)
+  const Dual5 p[3] = {pu[1] + fl * g.lx, pu[2] + fl * g.ly, pu[3] + fl * g.lz};
+  const float lam[6] = {l0, l1, l2, l3, l4, l5};
+  float out[5];
+  for (int k = 0; k < 5; ++k) {
+    float v = lE * E.d[k];
+    for (int c = 0; c < 3; ++c) v += lam[c] * ob[c].d[k] + lam[c + 3] * p[c].d[k];
+    out[k] = v;
+  }
+  return Grad5{out[0], out[1], out[2], out[3], out[4]};
+}
+
+__global__ void __launch_bounds__(64) k_vjp_backward(const SceneDev* __restrict__ scenes,
+                                                     uint32_t scene, VjpBufs vb,
+                                                     const float* __restrict__ grad_g,
+                                                     float* __restrict__ grad_dir,
+                                                     const VjpTangents* __restrict__ tg,
+                                                     double* __restrict__ grad_acc) {
+  const SceneDev& s = scenes[scene];
+  const SceneT<float> sc = scene_f32(s);
+  uint32_t px, py;
+  const bool valid = vjp_pixel(s, px, py);
+  const size_t rays = vb.rays;
+  const uint32_t idx = valid ? py * s.width + px : 0u;
+  const uint32_t N = valid ? vb.n[idx] : 0u;
+  const float gbar = valid ? grad_g[idx] : 0.0f;
+  float gp[3] = {0.0f, 0.0f, 0.0f};
+  float gd[2] = {0.0f, 0.0f};
+  if (N > 0u && gbar != 0.0f) {
+    const float E = vb.E[idx];
+    RhsBar<float> acc = {};
+    float y[6], lam[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) y[c] = vb.term[size_t(c) * rays + idx];
+    if (terminal_adjoint(sc, s, y, E, vb.h[size_t(N - 1) * rays + idx], gbar, lam, acc)) {
+      if (N >= 2u) {
+        for (int m = int((N - 2u) / vb.K); m >= 0; --m) {
+          const uint32_t n0 = uint32_t(m) * vb.K;
+          const uint32_t n1 = min(n0 + vb.K, N - 1u);
+          float yy[6];
+#pragma unroll
+          for (int c = 0; c < 6; ++c) yy[c] = vb.ckpt[(size_t(m) * 6 + c) * rays + idx];
+          for (uint32_t n = n0; n < n1; ++n) {
+#pragma unroll
+            for (int c = 0; c < 6; ++c) vb.seg[(size_t(n - n0) * 6 + c) * rays + idx] = yy[c];
+            advance<float>(sc, yy, E, vb.h[size_t(n) * rays + idx]);
+          }
+          for (uint32_t n = n1; n-- > n0;) {
+#pragma unroll
+            for (int c = 0; c < 6; ++c) yy[c] = vb.seg[(size_t(n - n0) * 6 + c) * rays + idx];
+            step_adjoint<float>(sc, yy, E, vb.h[size_t(n) * rays + idx], lam, acc);
+          }
+        }
+      }
+      const Grad5 g5 = initial_contract(s, tg, px, py, lam[0], lam[1], lam[2], lam[3], lam[4],
+                                        lam[5], acc.E);
+      gp[0] = g5.a + acc.spin + 2.0f * sc.spin * acc.spin2;
+      gp[1] = g5.q + acc.charge2 * tg->dq2_dq;
+      gp[2] = g5.incl;
+      gd[0] = g5.sx;
+      gd[1] = g5.sy;
+    }
+  }
+  if (valid && grad_dir) {
+    grad_dir[idx] = gd[0];
+    grad_dir[rays + idx] = gd[1];
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+    for (int off = 32; off > 0; off >>= 1) gp[k] += __shfl_xor(gp[k], off);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) atomicAdd(&grad_acc[k], double(gp[k]));
+  }
+}
+
+}  // namespace gl
+
+// ===========================================================================
+// Host side
+
+using namespace gl;
+
+struct gl_vjp_state {
+  gl_trace_plan* plan = nullptr;
+  uint32_t scene = 0;
+  uint32_t tiles = 0;
+  VjpBufs vb = {};
+  VjpTangents* d_tangents = nullptr;
+  double* d_grad = nullptr;
+  double* h_grad = nullptr;  // pinned
+};
+
+namespace {
+
+void release(gl_vjp_state* st) {
+  if (!st) return;
+  void* dev[] = {st->vb.h, st->vb.ckpt, st->vb.term, st->vb.E, st->vb.n, st->vb.seg,
+                 st->d_tangents, st->d_grad};
+  for (void* p : dev)
+    if (p) (void)hipFree(p);
+  if (st->h_grad) (void)hipHostFree(st->h_grad);
+  delete st;
+}
+
+// Observer state of a scene re-packed at (a, Q, inclination); the observer
+// keeps its oblate radius and azimuth.
+int observer_at(const gl_scene_desc& d, double r, double phi, double a, double q, double th,
+                SceneHost* out, std::string* err) {
+  gl_scene_desc e = d;
+  const double s = d.chart >= 0 ? 1.0 : -1.0;
+  e.spin = a;
+  e.charge = q;
+  e.observer_txyz[1] = (r * std::cos(phi) - s * a * std::sin(phi)) * std::sin(th);
+  e.observer_txyz[2] = (r * std::sin(phi) + s * a * std::cos(phi)) * std::sin(th);
+  e.observer_txyz[3] = r * std::cos(th);
+  SceneDev sd;
+  return prepare_scene(e, &sd, out, err);
+}
+
+// f64 central differences of the observer position and tetrad.
+gl_status observer_tangents(const SceneHost& hs, double inclination, VjpTangents* t) {
+  const gl_scene_desc& d = hs.desc;
+  const double s = d.chart >= 0 ? 1.0 : -1.0;
+  const double ox = d.observer_txyz[1], oy = d.observer_txyz[2], oz = d.observer_txyz[3];
+  const double r = geometry<double>(ox, oy, oz, d.mass, d.spin, d.spin * d.spin,
+                                    d.charge * d.charge, s).r;
+  const double th = std::acos(std::fmax(-1.0, std::fmin(1.0, oz / r)));
+  const double phi = std::atan2(oy, ox) - std::atan2(s * d.spin, r);
+  if (!(std::fabs(th - inclination) <= 1e-6))
+    return fail(GL_ERR_INVALID_ARGUMENT,
+                "inclination does not match the scene's observer position (polar angle " +
+                    std::to_string(th) + ")");
+  const double step = 1e-6;
+  for (int k = 0; k < 3; ++k) {
+    SceneHost hp, hm;
+    std::string err;
+    double ap = d.spin, am = d.spin, qp = d.charge, qm = d.charge, tp = th, tm = th;
+    if (k == 0) ap += step, am -= step;
+    if (k == 1) qp += step, qm -= step;
+    if (k == 2) tp += step, tm -= step;
+    if (observer_at(d, r, phi, ap, qp, tp, &hp, &err) || observer_at(d, r, phi, am, qm, tm, &hm, &err))
+      return fail(GL_ERR_DOMAIN, "perturbed scene for the observer tangents: " + err);
+    const double inv = 1.0 / (2.0 * step);
+    for (int c = 0; c < 3; ++c)
+      t->obs[k][c] = float((hp.desc.observer_txyz[c + 1] - hm.desc.observer_txyz[c + 1]) * inv);
+    t->ut[k] = float((hp.u[0] - hm.u[0]) * inv);
+    for (int c = 0; c < 4; ++c) {
+      t->right[k][c] = float((hp.right[c] - hm.right[c]) * inv);
+      t->up[k][c] = float((hp.up[c] - hm.up[c]) * inv);
+      t->arrival[k][c] = float((hp.arrival[c] - hm.arrival[c]) * inv);
+    }
+  }
+  t->dq2_dq = float(2.0 * d.charge);
+  return GL_OK;
+}
+
+}  // namespace
 
 extern "C" {
 
-gl_status gl_vjp_forward(gl_trace_plan*, uint32_t, double, uint32_t, const gl_ray_outputs*, void*,
+gl_status gl_vjp_forward(gl_trace_plan* p, uint32_t scene, double inclination,
+                         uint32_t checkpoint_interval, const gl_ray_outputs* out, void* stream,
                          gl_vjp_state** out_state) {
   if (out_state) *out_state = nullptr;
-  return GL_ERR_INVALID_ARGUMENT;
+  if (!p || !out || !out_state) return fail(GL_ERR_INVALID_ARGUMENT, "plan, outputs and out_state are required");
+  if (scene >= p->scenes.size()) return fail(GL_ERR_INVALID_ARGUMENT, "scene index out of range");
+  if (checkpoint_interval == 0 || checkpoint_interval > 1024)
+    return fail(GL_ERR_INVALID_ARGUMENT, "checkpoint_interval must lie in [1, 1024]");
+  const SceneDev& sd = p->scenes[scene];
+  const SceneHost& hs = p->host[scene];
+  if (!hs.desc.has_surface) return fail(GL_ERR_INVALID_ARGUMENT, "the adjoint needs a scene with a surface");
+  const size_t rays = size_t(sd.width) * sd.height;
+  if (out->count < rays) return fail(GL_ERR_INVALID_ARGUMENT, "output planes hold fewer rays than the scene");
+  VjpTangents tan;
+  gl_status s = observer_tangents(hs, inclination, &tan);
+  if (s) return s;
+  gl_vjp_state* st = new gl_vjp_state();
+  st->plan = p;
+  st->scene = scene;
+  st->tiles = sd.tiles_x * ((sd.height + 7u) / 8u);
+  VjpBufs& vb = st->vb;
+  vb.rays = uint32_t(rays);
+  vb.K = checkpoint_interval;
+  vb.cap = std::min<uint32_t>(sd.max_accepted, 1024u);
+  vb.ncp = (vb.cap + vb.K - 1u) / vb.K;
+  const size_t seg_steps = std::min<uint32_t>(vb.K, vb.cap);
+  hipError_t e = hipSuccess;
+  auto alloc = [&](void** ptr, size_t bytes) {
+    if (e == hipSuccess) e = hipMalloc(ptr, bytes);
+  };
+  alloc((void**)&vb.h, size_t(vb.cap) * rays * sizeof(float));
+  alloc((void**)&vb.ckpt, size_t(vb.ncp) * 6 * rays * sizeof(float));
+  alloc((void**)&vb.term, 6 * rays * sizeof(float));
+  alloc((void**)&vb.E, rays * sizeof(float));
+  alloc((void**)&vb.n, rays * sizeof(uint32_t));
+  alloc((void**)&vb.seg, seg_steps * 6 * rays * sizeof(float));
+  alloc((void**)&st->d_tangents, sizeof(VjpTangents));
+  alloc((void**)&st->d_grad, 3 * sizeof(double));
+  if (e == hipSuccess) e = hipHostMalloc((void**)&st->h_grad, 3 * sizeof(double), hipHostMallocDefault);
+  if (e == hipSuccess) e = hipMemcpy(st->d_tangents, &tan, sizeof(VjpTangents), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    release(st);
+    return fail(e == hipErrorOutOfMemory ? GL_ERR_CAPACITY : GL_ERR_DEVICE,
+                std::string("adjoint buffers: ") + hipGetErrorString(e));
+  }
+  hipStream_t hs_ = (hipStream_t)stream;
+  MapView map;
+  map.rgb = p->d_map;
+  map.w = p->map_w;
+  map.h = p->map_h;
+  OutPlanes planes;
+  planes.status = out->status;
+  planes.steps = out->steps;
+  planes.rejected = out->rejected;
+  planes.observable = out->observable;
+  planes.travel_time = out->travel_time;
+  planes.affine = out->affine;
+  planes.drift = out->drift;
+  planes.branch = out->branch;
+  planes.radiance = out->radiance;
+  planes.count = out->count;
+  plan_mark(p, 0, hs_);
+  hipLaunchKernelGGL(k_vjp_forward, dim3(st->tiles), dim3(64), 0, hs_, p->d_scenes, scene, planes,
+                     map, vb);
+  plan_mark(p, 1, hs_);
+  e = hipGetLastError();
+  if (e != hipSuccess) {
+    release(st);
+    return fail(GL_ERR_DEVICE, std::string("k_vjp_forward: ") + hipGetErrorString(e));
+  }
+  *out_state = st;
+  return GL_OK;
 }
 
-gl_status gl_vjp_backward(gl_vjp_state*, const float*, double*, float*, void*) {
-  return GL_ERR_INVALID_ARGUMENT;
+gl_status gl_vjp_backward(gl_vjp_state* st, const float* grad_g, double* grad_params,
+                          float* grad_direction, void* stream) {
+  if (!st || !grad_g || !grad_params)
+    return fail(GL_ERR_INVALID_ARGUMENT, "state, grad_g and grad_params are required");
+  hipStream_t hs_ = (hipStream_t)stream;
+  GL_HIP(hipMemsetAsync(st->d_grad, 0, 3 * sizeof(double), hs_));
+  plan_mark(st->plan, 0, hs_);
+  hipLaunchKernelGGL(k_vjp_backward, dim3(st->tiles), dim3(64), 0, hs_, st->plan->d_scenes,
+                     st->scene, st->vb, grad_g, grad_direction, st->d_tangents, st->d_grad);
+  plan_mark(st->plan, 1, hs_);
+  GL_HIP(hipGetLastError());
+  GL_HIP(hipMemcpyAsync(st->h_grad, st->d_grad, 3 * sizeof(double), hipMemcpyDeviceToHost, hs_));
+  GL_HIP(hipStreamSynchronize(hs_));
+  for (int k = 0; k < 3; ++k) grad_params[k] = st->h_grad[k];
+  return GL_OK;
 }
 
-gl_status gl_vjp_destroy(gl_vjp_state*) { return GL_OK; }
+gl_status gl_vjp_destroy(gl_vjp_state* st) {
+  release(st);
+  return GL_OK;
+}
 
 }  // extern "C"

@@ -56,6 +56,30 @@ namespace gl {
 
 // Quartic continuous extension of DP5 (Shampine 1986): weight of stage i at
 // fraction theta is theta * (d0 + theta * (d1 + theta * (d2 + theta * d3))).
+// dense_table holds the d coefficients (the same numbers dense_weights and
+// z_root spell out inline, where the f32 trace keeps them as literals).
+template <class T>
+GL_HD void dense_table(T d[7][4]) {
+  const T t[7][4] = {
+      {T(1.0), T(-8048581381.0 / 2820520608.0), T(8663915743.0 / 2820520608.0),
+       T(-12715105075.0 / 11282082432.0)},
+      {T(0), T(0), T(0), T(0)},
+      {T(0), T(131558114200.0 / 32700410799.0), T(-68118460800.0 / 10900136933.0),
+       T(87487479700.0 / 32700410799.0)},
+      {T(0), T(-1754552775.0 / 470086768.0), T(14199869525.0 / 1410260304.0),
+       T(-10690763975.0 / 1880347072.0)},
+      {T(0), T(127303824393.0 / 49829197408.0), T(-318862633887.0 / 49829197408.0),
+       T(701980252875.0 / 199316789632.0)},
+      {T(0), T(-282668133.0 / 205662961.0), T(2019193451.0 / 616988883.0),
+       T(-1453857185.0 / 822651844.0)},
+      {T(0), T(40617522.0 / 29380423.0), T(-110615467.0 / 29380423.0),
+       T(69997945.0 / 29380423.0)}};
+#pragma unroll
+  for (int i = 0; i < 7; ++i)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) d[i][k] = t[i][k];
+}
+
 template <class T>
 GL_HD void dense_weights(T th, T w[7]) {
   const T d[7][4] = {
@@ -74,6 +98,16 @@ GL_HD void dense_weights(T th, T w[7]) {
        T(69997945.0 / 29380423.0)}};
 #pragma unroll
   for (int i = 0; i < 7; ++i) w[i] = th * (d[i][0] + th * (d[i][1] + th * (d[i][2] + th * d[i][3])));
+}
+
+// d w_i / d theta.
+template <class T>
+GL_HD void dense_slopes(T th, T w[7]) {
+  T d[7][4];
+  dense_table<T>(d);
+#pragma unroll
+  for (int i = 0; i < 7; ++i)
+    w[i] = d[i][0] + th * (T(2) * d[i][1] + th * (T(3) * d[i][2] + th * (T(4) * d[i][3])));
 }
 
 // Scene constants in the working precision.

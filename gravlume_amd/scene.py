@@ -205,6 +205,15 @@ def small_cpu_scene(**kw) -> Scene:
     return Scene(**base)
 
 
+def adjoint_scene(**kw) -> Scene:
+    """512x512, a = 0.9, Q = 0.2, observer r = 100 M at 70 deg, FOV 20 deg,
+    thin disk [r_ISCO, 25 M]; the scene of the adjoint (VJP) workload."""
+    base = dict(spin=0.9, charge=0.2, inclination_deg=70.0, observer_radius=100.0, width=512,
+                height=512, fov_deg=20.0, disk=(None, 25.0), escape_map=False)
+    base.update(kw)
+    return Scene(**base)
+
+
 def uneven_scenes(count: int = 256, seed: int = 20261206, **kw) -> list[Scene]:
     """Seed-drawn scenes of uneven size: a ~ U[0, 0.998], Q ~ U[0, 0.9 sqrt(1 - a^2)],
     inclination ~ U[5, 89] deg, r_obs ~ U[30, 200] M, side in {128..1024},

@@ -11,6 +11,7 @@ PyTorch only provides device memory and the stream.
 from __future__ import annotations
 
 import ctypes as C
+import math
 from dataclasses import dataclass
 
 import numpy as np
@@ -270,3 +271,86 @@ def flop_constants() -> tuple[float, float]:
     a, b = C.c_double(), C.c_double()
     _capi.check(lib.gl_flop_constants(C.byref(a), C.byref(b)))
     return a.value, b.value
+
+
+class VjpTrace:
+    """Discrete-adjoint vector-Jacobian product of the per-pixel frequency
+    ratio g of one image scene (gl_vjp_forward / gl_vjp_backward).
+
+    The constructor runs the forward trace (outputs in ``self.target``) and
+    keeps the adjoint's saved state: the accepted step sizes, a checkpoint
+    every ``checkpoint_interval`` accepted steps, and the terminal state.
+    ``backward(grad_g)`` returns dL/d(a, Q, inclination) for
+    L = sum_pixels grad_g * g over surface hits, and, if asked, the per-pixel
+    gradient with respect to the sight-plane coordinates (s_x, s_y)."""
+
+    def __init__(self, pipeline: TracePipeline, scene: int = 0, checkpoint_interval: int = 32,
+                 stream=None):
+        self.pipeline = pipeline
+        self.scene = scene
+        sc = pipeline.scenes[scene]
+        self.target = pipeline.create_target(scene)
+        self._handle = C.c_void_p()
+        out = self.target.c_struct()
+        _capi.check(pipeline._lib.gl_vjp_forward(pipeline.handle, scene,
+                                                 math.radians(sc.inclination_deg),
+                                                 checkpoint_interval, C.byref(out),
+                                                 _stream_handle(stream), C.byref(self._handle)))
+
+    @property
+    def g(self) -> torch.Tensor:
+        """Per-pixel g (0 where the ray did not end on the surface)."""
+        st = self.target.status
+        hit = (st & 15) == 7
+        n = self.target.rays
+        return torch.where(hit, self.target.observable[2 * n:], torch.zeros_like(self.target.travel_time))
+
+    def backward(self, grad_g: torch.Tensor, direction: bool = False, stream=None):
+        grad_g = grad_g.detach().to(device=self.target.status.device, dtype=torch.float32).contiguous()
+        if grad_g.numel() != self.target.rays:
+            raise ValueError("grad_g needs one value per pixel")
+        gp = (C.c_double * 3)()
+        gdir = torch.zeros(2 * self.target.rays, dtype=torch.float32,
+                           device=grad_g.device) if direction else None
+        _capi.check(self.pipeline._lib.gl_vjp_backward(
+            self._handle, C.c_void_p(grad_g.data_ptr()), gp,
+            C.c_void_p(gdir.data_ptr()) if gdir is not None else None, _stream_handle(stream)))
+        params = np.array([gp[0], gp[1], gp[2]], dtype=np.float64)
+        if gdir is None:
+            return params
+        return params, gdir.view(2, -1)
+
+    def close(self) -> None:
+        if self._handle:
+            self.pipeline._lib.gl_vjp_destroy(self._handle)
+            self._handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001  (interpreter shutdown)
+            pass
+
+
+class FrequencyRatioImage(torch.autograd.Function):
+    """torch.autograd.Function over the adjoint trace: forward(params, scene)
+    with params = (a, Q, inclination [rad]) returns the per-pixel g image of
+    `scene` traced at those parameters; backward returns dL/dparams from the
+    discrete adjoint (the scene object is not differentiated)."""
+
+    @staticmethod
+    def forward(ctx, params: torch.Tensor, scene: Scene, checkpoint_interval: int = 32):
+        a, q, incl = (float(v) for v in params.detach().cpu().tolist())
+        sc = Scene(**{**scene.__dict__, "spin": a, "charge": q,
+                      "inclination_deg": math.degrees(incl)})
+        pipe = TracePipeline([sc])
+        vjp = VjpTrace(pipe, 0, checkpoint_interval)
+        ctx.vjp = vjp
+        ctx.pipe = pipe
+        return vjp.g.clone()
+
+    @staticmethod
+    def backward(ctx, grad_g):
+        gp = ctx.vjp.backward(grad_g)
+        ctx.vjp.close()
+        return torch.tensor(gp, dtype=torch.float64), None, None

@@ -220,7 +220,15 @@ gl_status gl_frame_graph_destroy(gl_frame_graph* graph);
 /* Discrete-adjoint vector-Jacobian product of per-pixel frequency ratios g
  * with respect to (spin a, charge Q, inclination, pixel direction), following
  * the torch.autograd.Function contract: forward saves state, backward takes
- * dL/dg per pixel. Scene `scene` must be an image scene with a surface. */
+ * dL/dg per pixel. Scene `scene` must be an image scene with a surface;
+ * `inclination` is the observer's polar angle in radians (checked against the
+ * scene's observer position, whose oblate radius and azimuth stay fixed).
+ * The forward writes the trace outputs to `out` and records the accepted
+ * step sizes (up to min(max_accepted_steps, 1024) per ray; longer rays get
+ * no gradient), a checkpoint every `checkpoint_interval` accepted steps and
+ * the terminal state; no f64 near-horizon deferral is used. The backward
+ * synchronizes `stream`. With plan timing enabled, both kernels are timed
+ * like a trace launch (gl_trace_plan_kernel_ms). */
 typedef struct gl_vjp_state gl_vjp_state;
 gl_status gl_vjp_forward(gl_trace_plan* plan, uint32_t scene, double inclination,
                          uint32_t checkpoint_interval, const gl_ray_outputs* out, void* stream,
