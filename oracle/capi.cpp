@@ -191,6 +191,18 @@ int orc_observation_trace_ordered(const Observation* obs, const Policy* pol, con
   });
 }
 
+// Forward-mode tangents of g for a batch of sight-plane points (tangent.cpp).
+int orc_tangent_g_ordered(double a, double q, double incl, double chart, double r_obs,
+                          uint32_t width, uint32_t height, double fov, const double* sight_xy,
+                          size_t n, double inner, double outer, double escape, double rtol,
+                          unsigned threads, double* g, double* dg, int* status) {
+  return parallel_ordered(n, threads, [&](size_t i) {
+    status[i] = tangent_frequency_ratio(a, q, incl, chart, r_obs, width, height, fov,
+                                        sight_xy[2 * i], sight_xy[2 * i + 1], inner, outer, escape,
+                                        rtol, &g[i], &dg[5 * i]);
+  });
+}
+
 double orc_isco_prograde(double spin, double charge) { return isco_prograde(spin, charge); }
 
 int orc_compare_extremality(double m, double a, double q) { return compare_extremality(m, a, q); }

@@ -210,4 +210,13 @@ int observable_at(const Spacetime& st, const Emitter& em, double inner, double o
 
 double isco_prograde(double spin, double charge);
 
+// Forward-mode tangents (tangent.cpp): g and dg/d(a, Q, inclination, s_x, s_y)
+// of the backward ray through sight-plane point (sx, sy) of the observation
+// at (a, Q, inclination) with the observer at oblate radius r_obs, azimuth 0,
+// looking at the origin with +z up. Returns 0 on a surface hit.
+int tangent_frequency_ratio(double a, double q, double incl, double chart, double r_obs,
+                            uint32_t width, uint32_t height, double fov, double sx, double sy,
+                            double inner, double outer, double escape, double rtol, double* g,
+                            double dg[5]);
+
 }  // namespace orc

@@ -124,6 +124,9 @@ def lib() -> C.CDLL:
         L.orc_isco_prograde.restype = C.c_double
         L.orc_compare_extremality.argtypes = [C.c_double] * 3
         L.orc_policy_preset.argtypes = [C.c_int, P(Policy)]
+        L.orc_tangent_g_ordered.argtypes = ([C.c_double] * 5 + [C.c_uint32, C.c_uint32,
+                                                                C.c_double, dp, C.c_size_t]
+                                            + [C.c_double] * 4 + [C.c_uint, dp, dp, P(C.c_int)])
         for name, struct in (("outcome", Outcome), ("observation", Observation),
                              ("policy", Policy), ("spacetime", Spacetime), ("events", Events)):
             fn = getattr(L, f"orc_sizeof_{name}")
@@ -317,6 +320,25 @@ def geometry(st, xyz):
 
 def isco_prograde(spin: float, charge: float) -> float:
     return lib().orc_isco_prograde(spin, charge)
+
+
+def tangent_frequency_ratio(spin, charge, inclination, chart, observer_radius, width, height,
+                            fov, sight_xy, surface, escape_radius, rtol=1e-10, threads=1):
+    """Forward-mode tangent integration (tangent.cpp): g and
+    dg/d(a, Q, inclination, s_x, s_y) per sight-plane point; status 0 marks
+    a surface hit."""
+    s = np.ascontiguousarray(sight_xy, dtype=np.float64).reshape(-1, 2)
+    n = s.shape[0]
+    g = np.zeros(n)
+    dg = np.zeros((n, 5))
+    st = np.zeros(n, dtype=np.int32)
+    rc = lib().orc_tangent_g_ordered(spin, charge, inclination, chart, observer_radius, width,
+                                     height, fov, _dp(s), n, surface[0], surface[1],
+                                     escape_radius, rtol, threads, _dp(g), _dp(dg),
+                                     st.ctypes.data_as(C.POINTER(C.c_int)))
+    if rc:
+        raise ValueError("thread count must lie in [1, 256]")
+    return g, dg, st
 
 
 def extremality(mass, spin, charge) -> int:

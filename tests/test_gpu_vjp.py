@@ -1,5 +1,6 @@
 """Discrete-adjoint VJP of the per-pixel frequency ratio g against f64 central
-differences of the oracle (needs an MI355X).
+differences of the oracle and against the oracle's forward-mode tangent
+integration (oracle/tangent.cpp) (needs an MI355X).
 
 Workload: 512x512, a = 0.9, Q = 0.2, inclination 70 deg, r_obs = 100 M, FOV
 20 deg, thin disk [r_ISCO, 25 M], f32 forward at rtol 1e-6, checkpoints every
@@ -96,6 +97,25 @@ def test_vjp_matches_oracle_central_differences(gl, oracle):
     print("direction rel: median", np.median(rel_d), "p90", np.quantile(rel_d, 0.9))
     assert np.median(rel_d) <= 1e-4
     assert np.quantile(rel_d, 0.9) <= 1e-3
+    # the same VJP against the oracle's forward-mode tangent integration
+    tg_g, tg_dg, tg_st = oracle.tangent_frequency_ratio(
+        sc.spin, sc.charge, math.radians(sc.inclination_deg), float(sc.chart),
+        sc.observer_radius, sc.width, sc.height, math.radians(sc.fov_deg), sxy[valid], disk,
+        sc.escape_radius, rtol=1e-10, threads=16)
+    ok_t = tg_st == 0
+    assert ok_t.mean() > 0.95
+    want_t = (w[ok_t, None] * tg_dg[ok_t, :3]).sum(axis=0)
+    got_t = vjp.backward(torch.where(
+        torch.isin(torch.arange(sc.rays, device="cuda"), torch.from_numpy(idx[ok_t]).cuda()),
+        grad_g, torch.zeros_like(grad_g)))
+    rel_t = np.abs(got_t - want_t) / np.abs(want_t)
+    print("params vs tangents", got_t, want_t, rel_t)
+    assert np.all(rel_t <= 1e-4), (got_t, want_t, rel_t)
+    rel_dt = np.abs(gd[ok_t] - tg_dg[ok_t, 3:]) / np.maximum(np.abs(tg_dg[ok_t, 3:]),
+                                                            1e-3 * np.abs(tg_dg[ok_t, 3:]).max())
+    print("direction vs tangents: median", np.median(rel_dt), "p90", np.quantile(rel_dt, 0.9))
+    assert np.median(rel_dt) <= 1e-4
+    assert np.quantile(rel_dt, 0.9) <= 1e-3
     # pixels off the surface or with zero weight get no gradient
     off = np.setdiff1d(np.arange(sc.rays), idx)
     assert np.all(gdir.cpu().numpy()[:, off] == 0)
