@@ -38,6 +38,7 @@ FLAG_INVALID_DENOMINATOR = 4
 FLAG_DRIFT_BUDGET = 8
 FLAG_TURNING_ORDER = 16
 FLAG_TIED_EVENTS = 32
+FLAG_CERTIFIED_CAPTURE = 64
 
 OPT_CAPTURE_CERTIFICATE = 1
 OPT_F64_HORIZON_STEPS = 2

@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "certificate.h"
 #include "gl_types.h"
 #include "trace_device.h"
 
@@ -79,12 +80,31 @@ __device__ __noinline__ Momentum image_momentum(const SceneDev& s, uint32_t px, 
   return m;
 }
 
+// The interval radial-capture certificate at the observer (certificate.h),
+// from the ray's constants of motion there. Out of line, once per ray.
+__device__ __noinline__ bool certify_capture(const SceneDev& s, float px, float py, float pz,
+                                             float E, float r_obs, float dr_future) {
+  const Geom<float> g = geometry<float>(s.obs_x, s.obs_y, s.obs_z, s.mass, s.spin, s.spin2,
+                                        s.charge2, s.chart);
+  const float L = s.obs_x * py - s.obs_y * px;
+  const float Q = carter<float>(g, s.obs_x, s.obs_y, s.obs_z, px, py, pz, E, s.spin2);
+  const float q = float(sqrt(s.charge2_d));
+  return capture_certified(s.mass, s.spin, q, s.r_horizon, r_obs,
+                           s.obs_x * s.obs_x + s.obs_y * s.obs_y, dr_future, E, L, Q);
+}
+
 __device__ __forceinline__ void init_image_ray(const SceneDev& s, const SceneT<float>& sc,
                                                uint32_t px, uint32_t py, RayT<float>& R) {
   Momentum m = image_momentum(s, px, py);
   init_ray<float>(sc, R, s.obs_x, s.obs_y, s.obs_z, 0.0f, 0.0f, 0.0f, m.px, m.py, m.pz, m.E,
                   -1.0f, s.polar_side);
   R.h = -s.h_init;
+  if ((sc.options & kOptCertificate) && R.term == kTermActive && R.flags == 0u &&
+      certify_capture(s, R.px, R.py, R.pz, R.E, R.r, R.rdot)) {
+    R.term = kTermHorizon;
+    R.candidates = 2u;
+    R.flags |= kFlagCertified;
+  }
 }
 
 __device__ void init_state_ray(const SceneDev& s, const SceneT<float>& sc, const double* y,

@@ -152,6 +152,10 @@ int prepare_scene(const gl_scene_desc& d, SceneDev* out, SceneHost* host, std::s
     return set_err(err, "step limits must be positive"), GL_ERR_INVALID_ARGUMENT;
   if (!(d.horizon_offset >= 0.0 && d.horizon_offset < 1.0))
     return set_err(err, "horizon offset must lie in [0, 1)"), GL_ERR_INVALID_ARGUMENT;
+  if ((d.options & GL_OPT_CAPTURE_CERTIFICATE) && d.has_surface)
+    return set_err(err, "the capture certificate applies to scenes without a surface: a certified "
+                        "ray could still cross the disk on its way in"),
+           GL_ERR_INVALID_ARGUMENT;
   if (d.has_surface) {
     if (!(d.surface_inner > 0.0 && d.surface_outer >= d.surface_inner))
       return set_err(err, "surface radii must be positive and ordered"), GL_ERR_INVALID_ARGUMENT;

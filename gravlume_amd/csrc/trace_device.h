@@ -195,6 +195,7 @@ enum : uint32_t {
   kFlagDrift = 8u,
   kFlagTurningOrder = 16u,
   kFlagTied = 32u,
+  kFlagCertified = 64u,  // horizon proven by the interval capture certificate
 };
 enum : uint32_t {
   kOptCertificate = 1u,

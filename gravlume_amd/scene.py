@@ -122,6 +122,7 @@ class Scene:
     reproject_interval: int = 16
     subpixel: tuple[float, float] = (0.5, 0.5)
     options: int = (_capi.OPT_F64_DISK_ROOT | _capi.OPT_F64_HORIZON_STEPS)
+    capture_certificate: bool = False  # interval radial-capture certificate
     escape_map: bool = True
     extra: dict = field(default_factory=dict)
 
@@ -148,6 +149,8 @@ class Scene:
         d.charge = self.charge
         d.chart = self.chart
         opts = self.options
+        if self.capture_certificate:
+            opts |= _capi.OPT_CAPTURE_CERTIFICATE
         if self.escape_map:
             opts |= _capi.OPT_ESCAPE_MAP
         d.options = opts

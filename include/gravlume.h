@@ -43,8 +43,7 @@ enum {
   GL_TERM_NUMERICAL_FAILURE = 5,
   GL_TERM_UNCERTAIN = 6,
   GL_TERM_SURFACE = 7,
-  GL_TERM_REJECT_EXHAUSTION = 8,
-  GL_TERM_CERTIFIED_CAPTURE = 9 /* horizon proven by the interval certificate */
+  GL_TERM_REJECT_EXHAUSTION = 8
 };
 
 /* Event-candidate bits (trace_protocol.wgsl:95-98, geodesic.rs:616-634). */
@@ -62,12 +61,17 @@ enum {
   GL_FLAG_INVALID_DENOMINATOR = 4u,
   GL_FLAG_DRIFT_BUDGET = 8u,     /* invariant drift above the scene limit    */
   GL_FLAG_TURNING_ORDER = 16u,   /* turning bracket overlaps the terminal    */
-  GL_FLAG_TIED_EVENTS = 32u      /* several event candidates within the tie  */
+  GL_FLAG_TIED_EVENTS = 32u,     /* several event candidates within the tie  */
+  GL_FLAG_CERTIFIED_CAPTURE = 64u /* horizon class proven by the interval
+                                     capture certificate, not integrated   */
 };
 
 /* Option bits of gl_scene_desc.options. */
 enum {
-  GL_OPT_CAPTURE_CERTIFICATE = 1u, /* interval radial-capture certificate      */
+  GL_OPT_CAPTURE_CERTIFICATE = 1u, /* interval radial-capture certificate
+                                      (scenes without a surface): proven
+                                      captures end at once as class HORIZON
+                                      with GL_FLAG_CERTIFIED_CAPTURE */
   GL_OPT_F64_HORIZON_STEPS = 2u,   /* f64 DP5 steps inside 1.05 r_+            */
   GL_OPT_F64_DISK_ROOT = 4u,       /* f64 equatorial root solve + observables  */
   GL_OPT_ESCAPE_MAP = 8u           /* look escape directions up in the map     */

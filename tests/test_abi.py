@@ -82,6 +82,7 @@ def test_plan_rejects_non_normalized_or_invalid_scenes(monkeypatch):
     assert "g_tt" in _plan_error(observer_radius=1.9, inclination_deg=90.0, spin=0.0, charge=0.0)
     assert "step policy" in _plan_error(minimum_step=1.0, maximum_step=0.5)
     assert "surface radii" in _plan_error(disk=(5.0, 4.0))
+    assert "without a surface" in _plan_error(capture_certificate=True)
 
 
 def test_mass_and_frequency_normalization_is_required(monkeypatch):

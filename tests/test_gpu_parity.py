@@ -260,3 +260,41 @@ def test_frame_graph_replays_match_eager(gl):
         assert np.mean(h["termination"] == he["termination"]) >= 0.9995
         pipe.encode_persistent([eager])  # eager work between replays
         torch.cuda.synchronize()
+
+
+def test_capture_certificate_on_the_default_kerr_view(gl, oracle):
+    """The interval capture certificate on the reference's default view
+    (a = 0.8, r = 30 M, 60 deg, FOV 45 deg, 1280x720, sky only): the
+    certified fraction matches the reference's 56,937 / 921,600 px; every
+    certified pixel is a horizon capture in the full f32 trace and in the f64
+    oracle; every other pixel traces exactly as without the certificate."""
+    from gravlume_amd import scene as S
+    import torch
+    kw = dict(spin=0.8, charge=0.0, inclination_deg=60.0, observer_radius=30.0, width=1280,
+              height=720, fov_deg=45.0, disk=None, escape_map=False)
+    on_sc, off_sc = S.Scene(capture_certificate=True, **kw), S.Scene(**kw)
+    on_pipe, off_pipe = gl.TracePipeline([on_sc]), gl.TracePipeline([off_sc])
+    on, off = on_pipe.create_target(0), off_pipe.create_target(0)
+    on_pipe.encode_batch(on)
+    off_pipe.encode_batch(off)
+    torch.cuda.synchronize()
+    hn, hf = on.to_host(), off.to_host()
+    cert = (hn["flags"] & 64) != 0
+    frac = cert.mean()
+    print("certified fraction", frac, cert.sum())
+    assert abs(frac - 56937 / 921600) <= 1e-3
+    assert np.all(hn["termination"][cert] == 1) and np.all(hn["candidates"][cert] == 2)
+    assert np.all(hn["steps"][cert] == 0)
+    assert np.all(hf["termination"][cert] == 1)
+    assert np.array_equal(hn["termination"][~cert], hf["termination"][~cert])
+    assert np.array_equal(hn["observable"][~cert], hf["observable"][~cert])
+    idx = np.random.default_rng(3).choice(np.flatnonzero(cert), 256, replace=False)
+    obs = oracle.observation(1.0, on_sc.spin, on_sc.charge, float(on_sc.chart),
+                             [0.0, *on_sc.observer_xyz()], [0, 0, 0, 0], [0, 0, 1], 1.0,
+                             on_sc.width, on_sc.height, math.radians(on_sc.fov_deg))
+    sxy = oracle.sight_planes(obs, idx % on_sc.width, idx // on_sc.width)
+    outs, st = oracle.observation_trace_ordered(
+        obs, oracle.config_policy(1e-10, 1e-12),
+        oracle.events(on_sc.escape_radius, None, horizon_offset=on_sc.horizon_offset), sxy,
+        None, 8)
+    assert np.all(oracle.outcomes_to_arrays(outs)["termination"] == 1)
