@@ -36,6 +36,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-vjp", action="store_true", help="skip the adjoint leg")
+    ap.add_argument("--no-near-critical", action="store_true",
+                    help="skip the near-critical batch leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--accuracy-rays", type=int, default=2048)
     ap.add_argument("--persistent-config", type=str, default=None,
@@ -175,6 +177,83 @@ def vjp_leg(g, S, torch, repeats=5):
             "loss": loss, "grad_a_q_incl": [float(x) for x in grads]}
 
 
+def near_critical_leg(g, S, torch, n=1_000_000, sample=1024, seed=20261208):
+    """Near-critical batch: a = 0.998, Q = 0.05, observer r = 200 M at 85 deg;
+    n requests at offsets delta ~ +-log-U[1e-8, 1e-3] from the critical curve
+    (f64 initial data), f32 DP5 rtol 1e-6 with the f64 steps inside
+    1.05 r_+; plain batch kernel against the persistent lane-refill queue;
+    class / winding / travel-time agreement with the f64 oracle at rtol
+    1e-12 on a seeded sample, by |delta| band."""
+    import math
+    from gravlume_amd import critical as Cr
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    sc = S.near_critical_scene()
+    pipe = g.TracePipeline([sc])
+    pipe.enable_timing(True)
+    states, delta, _, _ = Cr.near_critical_states(sc.spin, sc.charge, sc.observer_radius,
+                                                  math.radians(sc.inclination_deg), n, seed=seed,
+                                                  chart=sc.chart)
+    d_states = torch.from_numpy(states).cuda()
+    d_dirs = -torch.ones(n, dtype=torch.float32, device="cuda")
+    batch = g.GeodesicBatch(pipe, 0)
+    res = {"workload": f"{n} rays, Kerr-Newman a=0.998 Q=0.05, observer r=200M incl 85deg, "
+                       "signed offsets delta ~ log-U[1e-8,1e-3] from the critical curve, "
+                       "f32 DP5 rtol 1e-6 + f64 steps inside 1.05 r+"}
+    tgt = {}
+    for name, persistent in (("plain", False), ("persistent", True)):
+        t = g.TraceTarget(n)
+        batch.encode(d_states, d_dirs, t, persistent=persistent)  # warm-up
+        torch.cuda.synchronize()
+        pipe.kernel_ms()
+        t0 = time.perf_counter()
+        batch.encode(d_states, d_dirs, t, persistent=persistent)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        main_ms, fin_ms, _ = pipe.kernel_ms()
+        res[name] = {"rays_per_s": n / wall, "ms": 1e3 * wall, "kernel_ms": main_ms,
+                     "f64_finisher_ms": fin_ms}
+        tgt[name] = t
+    res["persistent_over_plain"] = res["persistent"]["rays_per_s"] / res["plain"]["rays_per_s"]
+    h = tgt["persistent"].to_host()
+    res["steps"] = {"accepted_mean": float(h["steps"].mean()), "accepted_max": int(h["steps"].max()),
+                    "rejected_mean": float(h["rejected"].mean())}
+    from oracle import oracle as O
+    idx = np.random.default_rng(seed + 1).choice(n, sample, replace=False)
+    cores = max(1, min(16, os.cpu_count() or 1))
+    st = O.spacetime(1.0, sc.spin, sc.charge, float(sc.chart))
+    ev = O.events(sc.escape_radius, None, horizon_offset=sc.horizon_offset)
+    dirs = -np.ones(sample)
+    ref = O.outcomes_to_arrays(O.trace_ordered(st, O.config_policy(1e-12, 1e-14), ev, states[idx],
+                                               dirs, threads=cores))
+    fine = O.outcomes_to_arrays(O.trace_ordered(st, O.config_policy(1e-12 / 16, 1e-14 / 16), ev,
+                                                states[idx], dirs, threads=cores))
+    work = O.outcomes_to_arrays(O.trace_ordered(st, O.config_policy(sc.rtol, sc.atol), ev,
+                                                states[idx], dirs, threads=cores))
+    ok = ref["termination"] == fine["termination"]
+    settled = work["termination"] == ref["termination"]
+    bands = {}
+    ad = np.abs(delta[idx])
+    for lo, hi in ((1e-8, 1e-6), (1e-6, 1e-5), (1e-5, 1e-3)):
+        m = ok & (ad >= lo) & (ad < hi)
+        same = h["termination"][idx][m] == ref["termination"][m]
+        f64_same = settled[m]
+        wind = h["azimuth_winding"][idx][m] == ref["azimuth_winding"][m]
+        f64_wind = work["azimuth_winding"][m] == ref["azimuth_winding"][m]
+        esc = m & (ref["termination"] == 2) & (h["termination"][idx] == 2)
+        tt = np.abs(h["travel_time"][idx][esc] / ref["travel_time"][esc] - 1.0)
+        tt64 = np.abs(work["travel_time"][esc] / ref["travel_time"][esc] - 1.0)
+        bands[f"{lo:g}-{hi:g}"] = {
+            "rays": int(m.sum()), "class_agreement": float(same.mean()) if m.any() else None,
+            "f64_at_rtol_1e-6_class_agreement": float(f64_same.mean()) if m.any() else None,
+            "winding_agreement": float(wind.mean()) if m.any() else None,
+            "f64_at_rtol_1e-6_winding_agreement": float(f64_wind.mean()) if m.any() else None,
+            "travel_time_rel_median": float(np.median(tt)) if esc.any() else None,
+            "f64_at_rtol_1e-6_travel_time_rel_median": float(np.median(tt64)) if esc.any() else None}
+    res["vs_f64_rtol_1e-12"] = {"sample": sample, "oracle_flagged": int((~ok).sum()),
+                                "by_abs_delta": bands}
+    return res
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -292,6 +371,8 @@ def main():
     }
     if not args.no_vjp:
         line["vjp"] = vjp_leg(g, S, torch)
+    if not args.no_near_critical:
+        line["near_critical"] = near_critical_leg(g, S, torch)
     if not args.no_cpu:
         base, err = cpu_baseline(scene, host, args.cpu_seconds, args.accuracy_rays)
         line["cpu_baseline"] = base

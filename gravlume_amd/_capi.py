@@ -102,6 +102,9 @@ SIGNATURES = {
     "gl_trace_plan_get_persistent_config": (C.c_int, [_P, C.POINTER(PersistentConfig)]),
     "gl_geodesic_batch_trace": (C.c_int, [_P, C.c_uint32, _P, _P, C.c_size_t,
                                           C.POINTER(RayOutputsC), _P]),
+    "gl_geodesic_batch_trace_persistent": (C.c_int, [_P, C.c_uint32, _P, _P, C.c_size_t,
+                                                     C.POINTER(PersistentConfig),
+                                                     C.POINTER(RayOutputsC), _P]),
     "gl_trace_plan_queue_stats": (C.c_int, [_P, C.POINTER(QueueStats), _P]),
     "gl_trace_plan_enable_timing": (C.c_int, [_P, C.c_int]),
     "gl_trace_plan_kernel_ms": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double),

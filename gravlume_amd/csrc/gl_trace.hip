@@ -130,11 +130,22 @@ __global__ void __launch_bounds__(256, kWaves) k_persistent(const SceneDev* __re
               omega = s.omega;
             }
             uint32_t local = uint32_t(item - f.first_item);
-            uint32_t px, py;
-            image_pixel(local, s.tiles_x, px, py);
-            if (px < s.width && py < s.height) {
-              idx = py * s.width + px;
-              init_image_ray(s, sc, px, py, R);
+            bool have = false;
+            if (f.states) {  // geodesic batch request
+              idx = local;
+              init_state_ray(s, sc, f.states + 8 * size_t(local),
+                             f.dirs[local] >= 0.0f ? 1.0f : -1.0f, R);
+              have = true;
+            } else {
+              uint32_t px, py;
+              image_pixel(local, s.tiles_x, px, py);
+              if (px < s.width && py < s.height) {
+                idx = py * s.width + px;
+                init_image_ray(s, sc, px, py, R);
+                have = true;
+              }
+            }
+            if (have) {
               active = true;
               if (R.term != kTermActive) {
                 write_ray(R, f.out, idx, s, map);
@@ -578,7 +589,8 @@ const void* persistent_kernel(bool single, uint32_t waves, bool tiles) {
 // Queue reset, persistent kernel and (when any scene asks for f64 horizon
 // steps) the f64 finisher, all on `st`, reading the frame table d_table.
 gl_status launch_persistent(gl_trace_plan* p, const FrameScene* d_table, size_t n_table,
-                            unsigned long long total, size_t total_rays, hipStream_t st) {
+                            unsigned long long total, size_t total_rays, hipStream_t st,
+                            const gl_persistent_config& c) {
   gl_status s = GL_OK;
   bool f64 = any_f64_horizon(p);
   if (f64) {
@@ -588,7 +600,6 @@ gl_status launch_persistent(gl_trace_plan* p, const FrameScene* d_table, size_t 
   hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, st, p->d_queue, total,
                      p->d_defer_count);
   GL_HIP(hipGetLastError());
-  const gl_persistent_config& c = p->pcfg;
   // Every workgroup must be resident at once: clamp the requested blocks
   // per CU to what the kernel's register budget allows.
   int resident = 0;
@@ -628,6 +639,20 @@ gl_status launch_persistent(gl_trace_plan* p, const FrameScene* d_table, size_t 
   return GL_OK;
 }
 
+gl_status check_config(const gl_persistent_config& c) {
+  if (c.block_size < 64 || c.block_size > 256 || c.block_size % 64 != 0)
+    return fail(GL_ERR_INVALID_ARGUMENT, "block_size must be 64, 128, 192 or 256");
+  if (c.blocks_per_cu == 0 || c.blocks_per_cu > 32)
+    return fail(GL_ERR_INVALID_ARGUMENT, "blocks_per_cu must lie in [1, 32]");
+  if (c.refill_interval == 0 || c.refill_interval > 4096)
+    return fail(GL_ERR_INVALID_ARGUMENT, "refill_interval must lie in [1, 4096]");
+  if (c.refill_threshold == 0 || c.refill_threshold > 64)
+    return fail(GL_ERR_INVALID_ARGUMENT, "refill_threshold must lie in [1, 64]");
+  if (c.waves_per_simd < 2 || c.waves_per_simd > 4)
+    return fail(GL_ERR_INVALID_ARGUMENT, "waves_per_simd must be 2, 3 or 4");
+  return GL_OK;
+}
+
 gl_status persistent_frame(gl_trace_plan* p, const uint32_t* scenes, size_t n_scenes,
                            const gl_ray_outputs* outputs, hipStream_t st) {
   std::vector<FrameScene> table;
@@ -637,7 +662,7 @@ gl_status persistent_frame(gl_trace_plan* p, const uint32_t* scenes, size_t n_sc
   if (s) return s;
   s = upload_frame(p, table, st);
   if (s) return s;
-  return launch_persistent(p, p->d_frame, table.size(), total, rays, st);
+  return launch_persistent(p, p->d_frame, table.size(), total, rays, st, p->pcfg);
 }
 
 }  // namespace
@@ -768,16 +793,8 @@ gl_status gl_trace_encode_persistent(gl_trace_plan* p, const uint32_t* scenes, s
 
 gl_status gl_trace_plan_set_persistent_config(gl_trace_plan* p, const gl_persistent_config* c) {
   if (!p || !c) return fail(GL_ERR_INVALID_ARGUMENT, "plan and config are required");
-  if (c->block_size < 64 || c->block_size > 256 || c->block_size % 64 != 0)
-    return fail(GL_ERR_INVALID_ARGUMENT, "block_size must be 64, 128, 192 or 256");
-  if (c->blocks_per_cu == 0 || c->blocks_per_cu > 32)
-    return fail(GL_ERR_INVALID_ARGUMENT, "blocks_per_cu must lie in [1, 32]");
-  if (c->refill_interval == 0 || c->refill_interval > 4096)
-    return fail(GL_ERR_INVALID_ARGUMENT, "refill_interval must lie in [1, 4096]");
-  if (c->refill_threshold == 0 || c->refill_threshold > 64)
-    return fail(GL_ERR_INVALID_ARGUMENT, "refill_threshold must lie in [1, 64]");
-  if (c->waves_per_simd < 2 || c->waves_per_simd > 4)
-    return fail(GL_ERR_INVALID_ARGUMENT, "waves_per_simd must be 2, 3 or 4");
+  gl_status s = check_config(*c);
+  if (s) return s;
   p->pcfg = *c;
   return GL_OK;
 }
@@ -787,6 +804,32 @@ gl_status gl_trace_plan_get_persistent_config(const gl_trace_plan* p, gl_persist
   *c = p->pcfg;
   return GL_OK;
 }
+
+gl_status gl_geodesic_batch_trace_persistent(gl_trace_plan* p, uint32_t scene,
+                                             const double* states8, const float* directions,
+                                             size_t n, const gl_persistent_config* config,
+                                             const gl_ray_outputs* out, void* stream) {
+  if (!p || !out) return fail(GL_ERR_INVALID_ARGUMENT, "plan and outputs are required");
+  if (scene >= p->scenes.size()) return fail(GL_ERR_INVALID_ARGUMENT, "scene index out of range");
+  const gl_persistent_config c = config ? *config : gl_persistent_config{256, 2, 8, 16, 2};
+  gl_status s = check_config(c);
+  if (s) return s;
+  if (c.refill_threshold >= 64u)
+    return fail(GL_ERR_INVALID_ARGUMENT, "geodesic batches refill lanes: refill_threshold must be below 64");
+  if (n == 0) return GL_OK;
+  if (!states8 || !directions) return fail(GL_ERR_INVALID_ARGUMENT, "states and directions are required");
+  if (n > 0xffffffffull) return fail(GL_ERR_CAPACITY, "more than 2^32 requests");
+  s = check_outputs(out, n);
+  if (s) return s;
+  hipStream_t st = (hipStream_t)stream;
+  FrameScene f = make_frame(scene, 0, uint32_t(n), *out);
+  f.states = states8;
+  f.dirs = directions;
+  s = upload_frame(p, {f}, st);
+  if (s) return s;
+  return launch_persistent(p, p->d_frame, 1, n, n, st, c);
+}
+
 
 gl_status gl_geodesic_batch_trace(gl_trace_plan* p, uint32_t scene, const double* states8,
                                   const float* directions, size_t n, const gl_ray_outputs* out,
@@ -892,7 +935,7 @@ gl_status gl_frame_graph_create(gl_trace_plan* p, const uint32_t* scenes, size_t
     return hip_fail(e, "hipStreamBeginCapture");
   }
   p->capturing = true;
-  s = launch_persistent(p, g->d_table, table.size(), total, rays, st);
+  s = launch_persistent(p, g->d_table, table.size(), total, rays, st, p->pcfg);
   p->capturing = false;
   hipGraph_t graph = nullptr;
   e = hipStreamEndCapture(st, &graph);

@@ -73,10 +73,14 @@ struct OutPlanes {
 // Per-scene slice of a persistent frame.
 struct FrameScene {
   uint32_t scene;       // index into the scene table
-  uint32_t first_item;  // prefix sum of rays
+  uint32_t first_item;  // prefix sum of queue items
   uint32_t rays;
   uint32_t pad;
   OutPlanes out;
+  // Geodesic batch: f64 initial states (8 per request) and affine directions;
+  // null for image scenes, whose items are 8x8-tile-ordered pixels.
+  const double* states;
+  const float* dirs;
 };
 
 constexpr int kMaxScenes = 4096;

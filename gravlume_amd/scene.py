@@ -217,6 +217,16 @@ def adjoint_scene(**kw) -> Scene:
     return Scene(**base)
 
 
+def near_critical_scene(**kw) -> Scene:
+    """Kerr-Newman a = 0.998, Q = 0.05, observer r = 200 M at 85 deg, no disk,
+    escape at 1000 M: the spacetime and policy of the near-critical batch
+    (initial states from gravlume_amd.critical)."""
+    base = dict(spin=0.998, charge=0.05, inclination_deg=85.0, observer_radius=200.0,
+                width=64, height=64, fov_deg=15.0, disk=None, escape_map=False)
+    base.update(kw)
+    return Scene(**base)
+
+
 def uneven_scenes(count: int = 256, seed: int = 20261206, **kw) -> list[Scene]:
     """Seed-drawn scenes of uneven size: a ~ U[0, 0.998], Q ~ U[0, 0.9 sqrt(1 - a^2)],
     inclination ~ U[5, 89] deg, r_obs ~ U[30, 200] M, side in {128..1024},

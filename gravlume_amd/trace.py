@@ -244,7 +244,9 @@ class GeodesicBatch:
         self.pipeline = pipeline
         self.scene = scene
 
-    def trace_ordered(self, states: np.ndarray, directions: np.ndarray, stream=None) -> dict:
+    def trace_ordered(self, states: np.ndarray, directions: np.ndarray, stream=None,
+                      persistent: bool = False, config: dict | None = None) -> dict:
+        """Host arrays in, decoded host outputs back (synchronizes)."""
         states = np.ascontiguousarray(states, dtype=np.float64).reshape(-1, 8)
         n = states.shape[0]
         dirs = np.ascontiguousarray(directions, dtype=np.float32).reshape(-1)
@@ -255,14 +257,32 @@ class GeodesicBatch:
             return target.to_host()
         d_states = torch.from_numpy(states).cuda()
         d_dirs = torch.from_numpy(dirs).cuda()
-        out = target.c_struct()
-        lib = self.pipeline._lib
-        _capi.check(lib.gl_geodesic_batch_trace(self.pipeline.handle, self.scene,
-                                                C.c_void_p(d_states.data_ptr()),
-                                                C.c_void_p(d_dirs.data_ptr()), n, C.byref(out),
-                                                _stream_handle(stream)))
+        self.encode(d_states, d_dirs, target, stream, persistent, config)
         torch.cuda.synchronize()
         return target.to_host()
+
+    def encode(self, d_states: torch.Tensor, d_dirs: torch.Tensor, target: TraceTarget,
+               stream=None, persistent: bool = False, config: dict | None = None) -> None:
+        """Device arrays in (float64 (n, 8) states, float32 (n,) directions),
+        outputs into `target`; enqueued on `stream`, no synchronization."""
+        n = d_states.shape[0]
+        out = target.c_struct()
+        lib = self.pipeline._lib
+        if persistent:
+            cfg = None
+            if config is not None:
+                cfg = C.byref(_capi.PersistentConfig(
+                    config.get("block_size", 256), config.get("blocks_per_cu", 2),
+                    config.get("refill_interval", 8), config.get("refill_threshold", 16),
+                    config.get("waves_per_simd", 2)))
+            _capi.check(lib.gl_geodesic_batch_trace_persistent(
+                self.pipeline.handle, self.scene, C.c_void_p(d_states.data_ptr()),
+                C.c_void_p(d_dirs.data_ptr()), n, cfg, C.byref(out), _stream_handle(stream)))
+        else:
+            _capi.check(lib.gl_geodesic_batch_trace(self.pipeline.handle, self.scene,
+                                                    C.c_void_p(d_states.data_ptr()),
+                                                    C.c_void_p(d_dirs.data_ptr()), n,
+                                                    C.byref(out), _stream_handle(stream)))
 
 
 def flop_constants() -> tuple[float, float]:

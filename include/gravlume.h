@@ -192,6 +192,17 @@ gl_status gl_geodesic_batch_trace(gl_trace_plan* plan, uint32_t scene,
                                   const double* states8, const float* directions, size_t n,
                                   const gl_ray_outputs* out, void* stream);
 
+/* The same batch through the persistent work queue: free lanes refill with
+ * new requests (lane-refill mode; config->refill_threshold < 64, NULL =
+ * {256, 2, 8, 16, 2}), so requests of very different lengths (near-critical
+ * rays) share wavefronts without waiting for the longest of 64. Results are
+ * identical in meaning to gl_geodesic_batch_trace. No reference counterpart
+ * beyond GeodesicBatch::trace_ordered (batch.rs:31-43). */
+gl_status gl_geodesic_batch_trace_persistent(gl_trace_plan* plan, uint32_t scene,
+                                             const double* states8, const float* directions,
+                                             size_t n, const gl_persistent_config* config,
+                                             const gl_ray_outputs* out, void* stream);
+
 /* Persistent-queue statistics of the last persistent launch. */
 typedef struct gl_queue_stats {
   uint64_t items;          /* rays handed out */
