@@ -38,6 +38,7 @@ def parse():
     ap.add_argument("--no-vjp", action="store_true", help="skip the adjoint leg")
     ap.add_argument("--no-near-critical", action="store_true",
                     help="skip the near-critical batch leg")
+    ap.add_argument("--no-uneven", action="store_true", help="skip the uneven multi-scene leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--accuracy-rays", type=int, default=2048)
     ap.add_argument("--persistent-config", type=str, default=None,
@@ -254,6 +255,67 @@ def near_critical_leg(g, S, torch, n=1_000_000, sample=1024, seed=20261208):
     return res
 
 
+def uneven_leg(g, S, torch, count=256, seed=20261206, jitter=0.1):
+    """Uneven multi-scene batch in one launch: `count` seed-drawn scenes
+    (a ~ U[0, 0.998], Q ~ U[0, 0.9 sqrt(1 - a^2)], inclination U[5, 89] deg,
+    r_obs U[30, 200] M, side in {128, 256, 512, 768, 1024}, FOV U[8, 30] deg,
+    disk [r_ISCO, 20 M], escape 1000 M), every pixel traced, plus per scene
+    `jitter` x its pixel count of extra rays seeded next to its shadow edge
+    (offsets +-log-U[1e-6, 1e-2] from the critical curve) as geodesic-batch
+    requests. One persistent launch over the whole work list (lane refill and
+    whole tiles) against one plain launch per image and per batch."""
+    scenes = S.uneven_scenes(count=count, seed=seed)
+    pipe = g.TracePipeline(scenes)
+    pipe.enable_timing(True)
+    items, keep = [], []
+    image_rays, batch_rays = 0, 0
+    per_scene = []
+    for i, sc in enumerate(scenes):
+        img = pipe.create_target(i)
+        items.append(("image", i, img))
+        states, _ = S.shadow_edge_requests(sc, jitter, seed=seed + 1 + i)
+        d_states = torch.from_numpy(states).cuda()
+        d_dirs = -torch.ones(len(states), dtype=torch.float32, device="cuda")
+        bt = g.TraceTarget(len(states))
+        items.append(("batch", i, d_states, d_dirs, bt))
+        keep.append((d_states, d_dirs, bt, img))
+        image_rays += sc.rays
+        batch_rays += len(states)
+        per_scene.append([sc.width, len(states)])
+    rays = image_rays + batch_rays
+    res = {"workload": f"{count} seed-drawn scenes (seed {seed}), {image_rays} pixel rays + "
+                       f"{batch_rays} shadow-edge rays, f32 DP5 rtol 1e-6, one launch",
+           "rays": rays}
+    for name, cfg in (("persistent_lane_refill", dict(refill_threshold=16, refill_interval=8)),
+                      ("persistent_whole_tile", dict(refill_threshold=64, refill_interval=8))):
+        pipe.set_persistent_config(block_size=256, blocks_per_cu=2, waves_per_simd=2, **cfg)
+        pipe.encode_work(items)  # warm-up
+        torch.cuda.synchronize()
+        pipe.kernel_ms()
+        t0 = time.perf_counter()
+        pipe.encode_work(items)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        main_ms, fin_ms, _ = pipe.kernel_ms()
+        st = pipe.queue_stats()
+        res[name] = {"rays_per_s": rays / wall, "ms": 1e3 * wall, "kernel_ms": main_ms,
+                     "f64_finisher_ms": fin_ms,
+                     "lane_utilization": st["attempted_steps"] / max(1, st["lane_slots"])}
+    # one plain launch per image and per batch, back to back on one stream
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i, (d_states, d_dirs, bt, img) in enumerate(keep):
+        pipe.encode_batch(img)
+        g.GeodesicBatch(pipe, i).encode(d_states, d_dirs, bt)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    res["plain_separate_launches"] = {"rays_per_s": rays / wall, "ms": 1e3 * wall}
+    best = max(res["persistent_lane_refill"]["rays_per_s"], res["persistent_whole_tile"]["rays_per_s"])
+    res["persistent_over_plain"] = best / res["plain_separate_launches"]["rays_per_s"]
+    res["per_scene_side_and_edge_rays"] = per_scene
+    return res
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -373,6 +435,8 @@ def main():
         line["vjp"] = vjp_leg(g, S, torch)
     if not args.no_near_critical:
         line["near_critical"] = near_critical_leg(g, S, torch)
+    if not args.no_uneven:
+        line["uneven_scenes"] = uneven_leg(g, S, torch)
     if not args.no_cpu:
         base, err = cpu_baseline(scene, host, args.cpu_seconds, args.accuracy_rays)
         line["cpu_baseline"] = base

@@ -78,6 +78,15 @@ class RayOutputsC(C.Structure):
                 ("count", C.c_size_t)]
 
 
+WORK_IMAGE = 0
+WORK_GEODESIC_BATCH = 1
+
+
+class WorkItem(C.Structure):
+    _fields_ = [("scene", C.c_uint32), ("kind", C.c_uint32), ("states8", C.c_void_p),
+                ("directions", C.c_void_p), ("count", C.c_size_t), ("outputs", RayOutputsC)]
+
+
 class PersistentConfig(C.Structure):
     _fields_ = [("block_size", C.c_uint32), ("blocks_per_cu", C.c_uint32),
                 ("refill_interval", C.c_uint32), ("refill_threshold", C.c_uint32),
@@ -102,6 +111,8 @@ SIGNATURES = {
     "gl_trace_plan_get_persistent_config": (C.c_int, [_P, C.POINTER(PersistentConfig)]),
     "gl_geodesic_batch_trace": (C.c_int, [_P, C.c_uint32, _P, _P, C.c_size_t,
                                           C.POINTER(RayOutputsC), _P]),
+    "gl_scene_validate": (C.c_int, [C.POINTER(SceneDesc)]),
+    "gl_trace_encode_work": (C.c_int, [_P, C.POINTER(WorkItem), C.c_size_t, _P]),
     "gl_geodesic_batch_trace_persistent": (C.c_int, [_P, C.c_uint32, _P, _P, C.c_size_t,
                                                      C.POINTER(PersistentConfig),
                                                      C.POINTER(RayOutputsC), _P]),

@@ -141,13 +141,16 @@ def _bisect(f, lo, hi, iters=200):
 
 
 def near_critical_states(a: float, q: float, r_obs: float, theta: float, count: int,
-                         seed: int, chart: int = -1, delta_range=(1e-8, 1e-3)):
+                         seed: int, chart: int = -1, delta_range=(1e-8, 1e-3),
+                         uniform_angle: bool = True):
     """`count` rays at signed offsets delta ~ +-log-U[delta_range] from the
     critical curve at uniformly drawn position angles psi about the curve's
     centre: the curve point at psi is solved exactly (bisection on the orbit
     radius), then moved radially by the factor 1 + delta. Returns
     (states (count, 8), delta (count,), xi, eta): delta > 0 lies outside the
-    shadow edge (the ray escapes after orbiting), delta < 0 inside."""
+    shadow edge (the ray escapes after orbiting), delta < 0 inside. With
+    uniform_angle=False the curve points are drawn uniformly in orbit radius
+    instead (no root solve; faster for large counts)."""
     if abs(a) < 1e-9:
         raise ValueError("near-critical data needs a != 0 (use the photon sphere for a = 0)")
     rng = np.random.default_rng(seed)
@@ -169,8 +172,12 @@ def near_critical_states(a: float, q: float, r_obs: float, theta: float, count: 
 
     psi_lo, psi_hi = psi_of(np.array([r_lo]))[0], psi_of(np.array([r_hi]))[0]
     psi = rng.uniform(-math.pi, math.pi, count)
-    upper = np.clip(np.abs(psi), min(psi_lo, psi_hi), max(psi_lo, psi_hi))
-    rp = _bisect(lambda r: psi_of(r) - upper, np.full(count, r_lo), np.full(count, r_hi), iters=64)
+    if uniform_angle:
+        upper = np.clip(np.abs(psi), min(psi_lo, psi_hi), max(psi_lo, psi_hi))
+        rp = _bisect(lambda r: psi_of(r) - upper, np.full(count, r_lo), np.full(count, r_hi),
+                     iters=64)
+    else:
+        rp = rng.uniform(r_lo, r_hi, count)
     alpha_c, beta_c, _ = _celestial(a, q, theta, rp)
     lo, hi = math.log(delta_range[0]), math.log(delta_range[1])
     delta = np.exp(rng.uniform(lo, hi, count)) * np.where(rng.uniform(size=count) < 0.5, -1.0, 1.0)

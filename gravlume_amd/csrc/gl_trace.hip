@@ -131,11 +131,13 @@ __global__ void __launch_bounds__(256, kWaves) k_persistent(const SceneDev* __re
             }
             uint32_t local = uint32_t(item - f.first_item);
             bool have = false;
-            if (f.states) {  // geodesic batch request
-              idx = local;
-              init_state_ray(s, sc, f.states + 8 * size_t(local),
-                             f.dirs[local] >= 0.0f ? 1.0f : -1.0f, R);
-              have = true;
+            if (f.states) {  // geodesic batch request (items beyond rays pad to 64)
+              if (local < f.rays) {
+                idx = local;
+                init_state_ray(s, sc, f.states + 8 * size_t(local),
+                               f.dirs[local] >= 0.0f ? 1.0f : -1.0f, R);
+                have = true;
+              }
             } else {
               uint32_t px, py;
               image_pixel(local, s.tiles_x, px, py);
@@ -211,13 +213,27 @@ __global__ void __launch_bounds__(256, kWaves) k_persistent_tiles(
     const SceneDev& s = scenes[f.scene];
     SceneT<float> sc = scene_f32(s);
     hold_in_vgprs(sc);
-    uint32_t px, py;
-    image_pixel(uint32_t(item - f.first_item), s.tiles_x, px, py);
-    uint32_t it = 0;
-    if (px < s.width && py < s.height) {
-      const uint32_t idx = py * s.width + px;
-      RayT<float> R;
-      init_image_ray(s, sc, px, py, R);
+    const uint32_t local = uint32_t(item - f.first_item);
+    uint32_t it = 0, idx = 0;
+    bool have = false;
+    RayT<float> R;
+    if (f.states) {  // geodesic batch: 64 consecutive requests (padded to 64)
+      if (local < f.rays) {
+        idx = local;
+        init_state_ray(s, sc, f.states + 8 * size_t(local), f.dirs[local] >= 0.0f ? 1.0f : -1.0f,
+                       R);
+        have = true;
+      }
+    } else {
+      uint32_t px, py;
+      image_pixel(local, s.tiles_x, px, py);
+      if (px < s.width && py < s.height) {
+        idx = py * s.width + px;
+        init_image_ray(s, sc, px, py, R);
+        have = true;
+      }
+    }
+    if (have) {
       bool deferred = false;
       for (it = 1; R.term == kTermActive; ++it) {
         if (wants_defer(sc, R) && try_defer(df, R, fs, idx)) {
@@ -561,6 +577,51 @@ gl_status build_frame_table(gl_trace_plan* p, const uint32_t* scenes, size_t n_s
   return GL_OK;
 }
 
+// Frame table of a mixed work list: image scenes (all pixels, 8x8-tile
+// ordered) and geodesic batches (requests padded to a multiple of 64 so that
+// every 64-item chunk lies inside one entry).
+gl_status build_work_table(gl_trace_plan* p, const gl_work_item* items, size_t n,
+                           std::vector<FrameScene>* table, unsigned long long* total_items,
+                           size_t* total_rays) {
+  if (!items || n == 0) return fail(GL_ERR_INVALID_ARGUMENT, "no work items");
+  table->resize(n);
+  unsigned long long total = 0;
+  size_t rays_sum = 0;
+  for (size_t i = 0; i < n; ++i) {
+    const gl_work_item& w = items[i];
+    if (w.scene >= p->scenes.size()) return fail(GL_ERR_INVALID_ARGUMENT, "scene index out of range");
+    const SceneDev& sd = p->scenes[w.scene];
+    size_t rays = 0;
+    unsigned long long cnt = 0;
+    if (w.kind == GL_WORK_IMAGE) {
+      rays = size_t(sd.width) * sd.height;
+      cnt = 64ull * sd.tiles_x * ((sd.height + 7) / 8);
+    } else if (w.kind == GL_WORK_GEODESIC_BATCH) {
+      if (w.count > 0 && (!w.states8 || !w.directions))
+        return fail(GL_ERR_INVALID_ARGUMENT, "geodesic batch item needs states and directions");
+      if (w.count > 0xffffffffull) return fail(GL_ERR_CAPACITY, "more than 2^32 requests");
+      rays = w.count;
+      cnt = (w.count + 63ull) / 64ull * 64ull;
+    } else {
+      return fail(GL_ERR_INVALID_ARGUMENT, "unknown work item kind");
+    }
+    gl_status s = check_outputs(&w.outputs, rays);
+    if (s) return s;
+    if (total + cnt > 0xffffffffull) return fail(GL_ERR_CAPACITY, "work list exceeds 2^32 items");
+    FrameScene f = make_frame(w.scene, uint32_t(total), uint32_t(rays), w.outputs);
+    if (w.kind == GL_WORK_GEODESIC_BATCH) {
+      f.states = w.states8;
+      f.dirs = w.directions;
+    }
+    (*table)[i] = f;
+    total += cnt;
+    rays_sum += rays;
+  }
+  *total_items = total;
+  *total_rays = rays_sum;
+  return GL_OK;
+}
+
 // Lane-refill kernels for waves_per_simd 2..4; whole-tile kernels (refill
 // threshold 64) for 2 and 3.
 const void* persistent_kernel(bool single, uint32_t waves, bool tiles) {
@@ -672,6 +733,16 @@ extern "C" {
 const char* gl_last_error(void) { return gl::g_last_error.c_str(); }
 
 const char* gl_version(void) { return "gravlume-mi355x 0.1 (gfx950, f32 DP5(4) trace)"; }
+
+gl_status gl_scene_validate(const gl_scene_desc* scene) {
+  if (!scene) return fail(GL_ERR_INVALID_ARGUMENT, "scene is required");
+  SceneDev sd;
+  SceneHost hs;
+  std::string err;
+  int st = prepare_scene(*scene, &sd, &hs, &err);
+  if (st != GL_OK) return fail(gl_status(st), err);
+  return GL_OK;
+}
 
 gl_status gl_trace_plan_create(const gl_scene_desc* scenes, size_t n_scenes,
                                const float* escape_map_rgb, uint32_t map_width,
@@ -803,6 +874,20 @@ gl_status gl_trace_plan_get_persistent_config(const gl_trace_plan* p, gl_persist
   if (!p || !c) return fail(GL_ERR_INVALID_ARGUMENT, "plan and config are required");
   *c = p->pcfg;
   return GL_OK;
+}
+
+gl_status gl_trace_encode_work(gl_trace_plan* p, const gl_work_item* items, size_t n_items,
+                               void* stream) {
+  if (!p) return fail(GL_ERR_INVALID_ARGUMENT, "plan is required");
+  std::vector<FrameScene> table;
+  unsigned long long total = 0;
+  size_t rays = 0;
+  gl_status s = build_work_table(p, items, n_items, &table, &total, &rays);
+  if (s) return s;
+  hipStream_t st = (hipStream_t)stream;
+  s = upload_frame(p, table, st);
+  if (s) return s;
+  return launch_persistent(p, p->d_frame, table.size(), total, rays, st, p->pcfg);
 }
 
 gl_status gl_geodesic_batch_trace_persistent(gl_trace_plan* p, uint32_t scene,

@@ -9,6 +9,7 @@ adaptive DP5(4) policy.
 """
 from __future__ import annotations
 
+import ctypes as C
 import math
 from dataclasses import dataclass, field
 
@@ -230,18 +231,41 @@ def near_critical_scene(**kw) -> Scene:
 def uneven_scenes(count: int = 256, seed: int = 20261206, **kw) -> list[Scene]:
     """Seed-drawn scenes of uneven size: a ~ U[0, 0.998], Q ~ U[0, 0.9 sqrt(1 - a^2)],
     inclination ~ U[5, 89] deg, r_obs ~ U[30, 200] M, side in {128..1024},
-    FOV ~ U[8, 30] deg; disk [r_ISCO, 30 M]."""
+    FOV ~ U[8, 30] deg; disk [r_ISCO, 20 M], escape at 1000 M. A draw that
+    fails the scene checks (gl_scene_validate) is discarded and drawn again."""
     rng = np.random.default_rng(seed)
     sides = [128, 256, 512, 768, 1024]
+    lib = _capi.load()
     out = []
-    for _ in range(count):
+    while len(out) < count:
         a = float(rng.uniform(0.0, 0.998))
         q = float(rng.uniform(0.0, math.sqrt(1.0 - a * a) * 0.9))
         side = int(rng.choice(sides))
         s = Scene(spin=a, charge=q, inclination_deg=float(rng.uniform(5.0, 89.0)),
                   observer_radius=float(rng.uniform(30.0, 200.0)), width=side, height=side,
-                  fov_deg=float(rng.uniform(8.0, 30.0)), disk=(None, 30.0))
+                  fov_deg=float(rng.uniform(8.0, 30.0)), disk=(None, 20.0), escape_map=False)
         for k, v in kw.items():
             setattr(s, k, v)
+        d = s.to_desc()
+        if lib.gl_scene_validate(C.byref(d)) != 0:
+            continue  # e.g. an observer frame outside the 1e-12 Gram budget: draw again
         out.append(s)
     return out
+
+
+def shadow_edge_requests(scene: Scene, fraction: float, seed: int,
+                         delta_range=(1e-6, 1e-2)) -> tuple[np.ndarray, np.ndarray]:
+    """The jittered share of a scene's rays as geodesic-batch requests: a
+    `fraction` of its pixel count, each a backward ray from the scene's
+    observer at a signed offset delta ~ +-log-U[delta_range] from the
+    critical curve (the shadow edge), so its step count runs from tens to
+    thousands. Returns (states (n, 8) f64, delta (n,))."""
+    from .critical import near_critical_states
+    n = int(round(fraction * scene.rays))
+    if n == 0:
+        return np.zeros((0, 8)), np.zeros(0)
+    states, delta, _, _ = near_critical_states(scene.spin, scene.charge, scene.observer_radius,
+                                               math.radians(scene.inclination_deg), n, seed,
+                                               chart=scene.chart, delta_range=delta_range,
+                                               uniform_angle=False)
+    return states, delta

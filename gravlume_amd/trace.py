@@ -42,9 +42,10 @@ class TileRegion:
 class TraceTarget:
     """Caller-owned structure-of-arrays outputs for n rays on the device."""
 
-    def __init__(self, rays: int, device: str | torch.device = "cuda"):
+    def __init__(self, rays: int, device: str | torch.device = "cuda", scene: int | None = None):
         dev = torch.device(device)
         self.rays = rays
+        self.scene = scene  # the image scene it was created for (None: a batch)
         self.status = torch.zeros(rays, dtype=torch.int32, device=dev)
         self.steps = torch.zeros(rays, dtype=torch.int32, device=dev)
         self.rejected = torch.zeros(rays, dtype=torch.int32, device=dev)
@@ -136,14 +137,16 @@ class TracePipeline:
         return n.value
 
     def create_target(self, scene: int = 0, device="cuda") -> TraceTarget:
-        return TraceTarget(self.scene_rays(scene), device)
+        return TraceTarget(self.scene_rays(scene), device, scene=scene)
 
     def encode_batch(self, target: TraceTarget, region: TileRegion | None = None,
                      stream=None) -> None:
-        """Plain one-ray-per-lane trace of a tile region (encode_batch)."""
+        """Plain one-ray-per-lane trace of a tile region (encode_batch); by
+        default every tile of the scene the target was created for."""
         if region is None:
-            s = self.scenes[0]
-            region = TileRegion.full(0, s.width, s.height)
+            k = target.scene if target.scene is not None else 0
+            s = self.scenes[k]
+            region = TileRegion.full(k, s.width, s.height)
         r = _capi.TileRegionC(region.scene, region.tile_x, region.tile_y, region.tiles_x,
                               region.tiles_y)
         out = target.c_struct()
@@ -182,6 +185,27 @@ class TracePipeline:
         _capi.check(self._lib.gl_trace_plan_queue_stats(self._handle, C.byref(s),
                                                         _stream_handle(stream)))
         return {k: getattr(s, k) for k, _ in s._fields_}
+
+    def encode_work(self, items: list, stream=None) -> None:
+        """One persistent launch over a mixed work list. Each item is
+        ("image", scene, TraceTarget) or ("batch", scene, d_states (n, 8) f64,
+        d_dirs (n,) f32, TraceTarget), device tensors kept alive by the
+        caller until the launch completes."""
+        arr = (_capi.WorkItem * len(items))()
+        for i, it in enumerate(items):
+            w = arr[i]
+            if it[0] == "image":
+                w.scene, w.kind, w.count = it[1], _capi.WORK_IMAGE, 0
+                w.outputs = it[2].c_struct()
+            elif it[0] == "batch":
+                _, scene, d_states, d_dirs, tgt = it
+                w.scene, w.kind, w.count = scene, _capi.WORK_GEODESIC_BATCH, d_states.shape[0]
+                w.states8, w.directions = d_states.data_ptr(), d_dirs.data_ptr()
+                w.outputs = tgt.c_struct()
+            else:
+                raise ValueError(f"unknown work item kind {it[0]!r}")
+        _capi.check(self._lib.gl_trace_encode_work(self._handle, arr, len(items),
+                                                   _stream_handle(stream)))
 
     def enable_timing(self, enable: bool = True) -> None:
         """HIP events around the main kernel and the f64 finisher of each launch."""

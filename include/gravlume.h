@@ -145,6 +145,11 @@ typedef struct gl_ray_outputs {
 
 typedef struct gl_trace_plan gl_trace_plan;
 
+/* The scene checks of gl_trace_plan_create for one scene, on the host only
+ * (no device needed): GL_OK or the status and gl_last_error() text the plan
+ * would fail with (input.rs:45-131, observer.rs:67-183). */
+gl_status gl_scene_validate(const gl_scene_desc* scene);
+
 /* Validates and packs n_scenes scenes (TracePipeline::new, input.rs:45-131)
  * and uploads them with the escape map (width x height texels, 3 float
  * planes on the host, may be NULL). */
@@ -167,6 +172,21 @@ gl_status gl_trace_encode(gl_trace_plan* plan, const gl_tile_region* region,
 gl_status gl_trace_encode_persistent(gl_trace_plan* plan, const uint32_t* scenes,
                                      size_t n_scenes, const gl_ray_outputs* outputs,
                                      void* stream);
+
+/* Mixed work list for one persistent launch: whole image scenes and
+ * geodesic batches of any scenes of the plan, all drawn from the same work
+ * queue (uneven multi-scene batches in one launch). */
+enum { GL_WORK_IMAGE = 0u, GL_WORK_GEODESIC_BATCH = 1u };
+typedef struct gl_work_item {
+  uint32_t scene;
+  uint32_t kind;            /* GL_WORK_IMAGE or GL_WORK_GEODESIC_BATCH */
+  const double* states8;    /* batch: n x (t, x, y, z, p_t, p_x, p_y, p_z), device */
+  const float* directions;  /* batch: n affine directions, device */
+  size_t count;             /* batch: n requests (ignored for images) */
+  gl_ray_outputs outputs;   /* image: width*height rays; batch: n, request order */
+} gl_work_item;
+gl_status gl_trace_encode_work(gl_trace_plan* plan, const gl_work_item* items, size_t n_items,
+                               void* stream);
 
 /* Launch parameters of the persistent kernel (set by the autotuner). */
 typedef struct gl_persistent_config {

@@ -38,7 +38,7 @@ def test_struct_layouts_match_the_header(tmp_path):
     from gravlume_amd import _capi
     structs = {"gl_scene_desc": _capi.SceneDesc, "gl_ray_outputs": _capi.RayOutputsC,
                "gl_tile_region": _capi.TileRegionC, "gl_persistent_config": _capi.PersistentConfig,
-               "gl_queue_stats": _capi.QueueStats}
+               "gl_queue_stats": _capi.QueueStats, "gl_work_item": _capi.WorkItem}
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void){"]
     for cname, py in structs.items():
         lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
@@ -124,3 +124,23 @@ def test_product_isco_matches_oracle(oracle):
     for a, q in [(0.94, 0.0), (0.5, 0.3), (0.9, 0.2), (0.92, 0.18), (0.998, 0.05), (0.0, 1.0),
                  (0.3, 0.9), (0.0, 0.0)]:
         assert abs(isco_prograde(a, q) - oracle.isco_prograde(a, q)) <= 1e-12
+
+
+def test_scene_validate_needs_no_device_and_matches_the_oracle_frame_check(oracle):
+    """gl_scene_validate runs the plan's scene checks on the host. An observer
+    frame that exceeds the 1e-12 Gram budget is rejected by it exactly as by
+    the oracle's restatement of the reference frame construction."""
+    import math
+    from gravlume_amd import _capi, scene as S
+    lib = _capi.load()
+    good = S.small_cpu_scene(escape_map=False)
+    assert lib.gl_scene_validate(C.byref(good.to_desc())) == 0
+    bad = S.Scene(spin=0.08186844760218794, charge=0.7823507233737966,
+                  inclination_deg=74.81317707591471, observer_radius=196.28397290333865,
+                  width=512, height=512, fov_deg=20.0, disk=(None, 20.0), escape_map=False)
+    assert lib.gl_scene_validate(C.byref(bad.to_desc())) == 2
+    assert b"Gram" in lib.gl_last_error()
+    with pytest.raises(ValueError):
+        oracle.observation(1.0, bad.spin, bad.charge, float(bad.chart), [0.0, *bad.observer_xyz()],
+                           [0, 0, 0, 0], [0, 0, 1], 1.0, bad.width, bad.height,
+                           math.radians(bad.fov_deg))

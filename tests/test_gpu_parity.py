@@ -298,3 +298,49 @@ def test_capture_certificate_on_the_default_kerr_view(gl, oracle):
         oracle.events(on_sc.escape_radius, None, horizon_offset=on_sc.horizon_offset), sxy,
         None, 8)
     assert np.all(oracle.outcomes_to_arrays(outs)["termination"] == 1)
+
+
+@pytest.mark.parametrize("mode", [dict(refill_threshold=16, waves_per_simd=2),
+                                  dict(refill_threshold=64, waves_per_simd=2)],
+                         ids=["lane-refill", "whole-tile"])
+def test_mixed_work_list_matches_separate_launches(gl, mode):
+    """One persistent launch over image scenes of uneven size plus, per scene,
+    10% extra rays seeded next to its shadow edge as geodesic-batch requests
+    (the uneven multi-scene workload in miniature) agrees with tracing each
+    image and each batch in its own plain launch."""
+    from gravlume_amd import scene as S
+    import torch
+    scenes = S.uneven_scenes(count=6, seed=11)
+    for k, s in enumerate(scenes):
+        s.width = s.height = 48 + 16 * k
+    pipe = gl.TracePipeline(scenes)
+    pipe.set_persistent_config(block_size=256, blocks_per_cu=2, refill_interval=8, **mode)
+    items, images, batches, keep = [], [], [], []
+    for i, s in enumerate(scenes):
+        img = pipe.create_target(i)
+        items.append(("image", i, img))
+        images.append(img)
+        states, delta = S.shadow_edge_requests(s, 0.1, seed=100 + i)
+        d_states = torch.from_numpy(states).cuda()
+        d_dirs = -torch.ones(len(states), dtype=torch.float32, device="cuda")
+        bt = gl.TraceTarget(len(states))
+        items.append(("batch", i, d_states, d_dirs, bt))
+        batches.append((bt, states, delta))
+        keep.append((d_states, d_dirs))
+    pipe.encode_work(items)
+    torch.cuda.synchronize()
+    stats = pipe.queue_stats()
+    assert 0 < stats["attempted_steps"] <= stats["lane_slots"]
+    for i, s in enumerate(scenes):
+        ref = pipe.create_target(i)
+        pipe.encode_batch(ref)
+        torch.cuda.synchronize()
+        hi, hr = images[i].to_host(), ref.to_host()
+        assert np.all(hi["termination"] > 0)
+        assert np.mean(hi["termination"] == hr["termination"]) >= 0.999
+        bt, states, delta = batches[i]
+        hb = bt.to_host()
+        hp = gl.GeodesicBatch(pipe, i).trace_ordered(states, -np.ones(len(states)))
+        assert np.all(hb["termination"] > 0)
+        big = np.abs(delta) >= 1e-4
+        assert np.array_equal(hb["termination"][big], hp["termination"][big])
