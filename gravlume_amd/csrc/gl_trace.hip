@@ -262,7 +262,12 @@ __global__ void __launch_bounds__(64) k_finish_f64(const SceneDev* __restrict__ 
                                                    MapView map, DeferBuf df) {
   unsigned int n = *df.count;
   if (n > df.capacity) n = df.capacity;
-  for (unsigned int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+  // Each lane takes the next deferred ray as soon as its last one finishes
+  // (df.count[1] is the claim cursor), so long f64 tails do not hold up the
+  // rays queued behind them on the same lane.
+  for (;;) {
+    const unsigned int i = atomicAdd(df.count + 1, 1u);
+    if (i >= n) break;
     const DeferRecord& rec = df.rec[i];
     const FrameScene& f = frame[rec.frame_scene];
     const SceneDev& s = scenes[f.scene];
@@ -344,7 +349,10 @@ __global__ void k_queue_reset(QueueDev* q, unsigned long long total, unsigned in
     q->refills = 0;
     q->attempts = 0;
     q->slots = 0;
-    if (defer_count) *defer_count = 0;
+    if (defer_count) {
+      defer_count[0] = 0;  // deferred rays
+      defer_count[1] = 0;  // finisher claim cursor
+    }
   }
 }
 

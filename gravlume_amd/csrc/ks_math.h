@@ -71,9 +71,36 @@ GL_HD float gl_rcp<float>(float v) {
 #endif
 }
 
+// f64 on the device: the hardware v_rcp_f64 / v_rsq_f64 estimate refined by
+// two Newton steps (a few ulp, for finite nonzero v) instead of the IEEE
+// division and square-root sequences.
+template <>
+GL_HD double gl_rcp<double>(double v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  double r = __builtin_amdgcn_rcp(v);
+  r = __builtin_fma(__builtin_fma(-v, r, 1.0), r, r);
+  r = __builtin_fma(__builtin_fma(-v, r, 1.0), r, r);
+  return r;
+#else
+  return 1.0 / v;
+#endif
+}
+
 template <class T>
 GL_HD T gl_rsqrt(T v) {
   return T(1) / gl_sqrt<T>(v);
+}
+template <>
+GL_HD double gl_rsqrt<double>(double v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  double y = __builtin_amdgcn_rsq(v);
+  const double hv = 0.5 * v;
+  y = y * __builtin_fma(-hv * y, y, 1.5);
+  y = y * __builtin_fma(-hv * y, y, 1.5);
+  return y;
+#else
+  return 1.0 / __builtin_sqrt(v);
+#endif
 }
 template <>
 GL_HD float gl_rsqrt<float>(float v) {

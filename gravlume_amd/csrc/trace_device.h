@@ -259,7 +259,12 @@ GL_HD float pow_m02<float>(float v) {
 }
 template <>
 GL_HD double pow_m02<double>(double v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  // the step-size factor needs no f64 accuracy
+  return double(exp2f(-0.2f * log2f(float(v))));
+#else
   return pow(v, -0.2);
+#endif
 }
 
 template <class T>
