@@ -219,7 +219,7 @@ __device__ __forceinline__ void write_ray(const RayT<float>& R, const OutPlanes&
 // they would occupy some twenty SGPRs next to the loop's exec masks, and the
 // scalar file then spills to VGPR lanes with a readlane per use.
 __device__ __forceinline__ void hold_in_vgprs(SceneT<float>& sc) {
-  asm volatile("" : "+v"(sc.mass), "+v"(sc.spin), "+v"(sc.spin2), "+v"(sc.charge2), "+v"(sc.chart));
+  asm volatile("" : "+v"(sc.spin), "+v"(sc.spin2), "+v"(sc.charge2), "+v"(sc.chart));
   asm volatile("" : "+v"(sc.r_capture), "+v"(sc.r_horizon), "+v"(sc.r_escape), "+v"(sc.disk_in),
                "+v"(sc.disk_out));
   asm volatile("" : "+v"(sc.band), "+v"(sc.rtol), "+v"(sc.atol), "+v"(sc.h_min), "+v"(sc.h_max),

@@ -38,6 +38,15 @@ template <class T>
 GL_HD T gl_abs(T v) {
   return v < T(0) ? -v : v;
 }
+// |v| as a source modifier of the consuming instruction (no instruction).
+template <>
+GL_HD float gl_abs<float>(float v) {
+  return __builtin_fabsf(v);
+}
+template <>
+GL_HD double gl_abs<double>(double v) {
+  return __builtin_fabs(v);
+}
 
 // Square root, reciprocal and reciprocal square root. On the f32 device
 // path these are the hardware v_sqrt_f32 / v_rcp_f32 / v_rsq_f32 (about one

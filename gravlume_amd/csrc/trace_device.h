@@ -119,9 +119,12 @@ struct SceneT {
   uint32_t max_accepted, max_rejects, options, reproject_interval;
 };
 
+// The packed scene's mass is exactly 1 (the plan rejects any other), so the
+// mass is the literal 1 here and its products fold away in the kernels.
 GL_HD SceneT<float> scene_f32(const SceneDev& s) {
   SceneT<float> c;
-  c.mass = s.mass;
+  (void)s.mass;
+  c.mass = 1.0f;
   c.spin = s.spin;
   c.spin2 = s.spin2;
   c.charge2 = s.charge2;
@@ -149,7 +152,7 @@ GL_HD SceneT<float> scene_f32(const SceneDev& s) {
 
 GL_HD SceneT<double> scene_f64(const SceneDev& s) {
   SceneT<double> c;
-  c.mass = s.mass;
+  c.mass = 1.0;
   c.spin = s.spin_d;
   c.spin2 = s.spin_d * s.spin_d;
   c.charge2 = s.charge2_d;
@@ -303,13 +306,36 @@ template <class T>
 GL_HD T gl_clamp(T v, T lo, T hi) {
   return v < lo ? lo : (v > hi ? hi : v);
 }
+// One v_max / v_min (v_med3 for the clamp) instead of a compare and select;
+// these return the other operand for a NaN, so the step checks finiteness
+// of its new state explicitly.
+template <>
+GL_HD float gl_clamp<float>(float v, float lo, float hi) {
+  return __builtin_fminf(__builtin_fmaxf(v, lo), hi);
+}
 template <class T>
 GL_HD T gl_max(T a, T b) {
   return a > b ? a : b;
 }
+template <>
+GL_HD float gl_max<float>(float a, float b) {
+  return __builtin_fmaxf(a, b);
+}
+template <>
+GL_HD double gl_max<double>(double a, double b) {
+  return __builtin_fmax(a, b);
+}
 template <class T>
 GL_HD T gl_min(T a, T b) {
   return a < b ? a : b;
+}
+template <>
+GL_HD float gl_min<float>(float a, float b) {
+  return __builtin_fminf(a, b);
+}
+template <>
+GL_HD double gl_min<double>(double a, double b) {
+  return __builtin_fmin(a, b);
 }
 
 // RHS and geometry at (x, p). Returns false on a non-finite or singular
@@ -701,7 +727,9 @@ GL_HD void step(const SceneT<T>& sc, RayT<T>& R, T omega, bool reproject = false
            gl_rcp<T>(sc.atol + sc.rtol * gl_max<T>(gl_abs<T>(y0[c + 3]), gl_abs<T>(pn[c])));
     en = gl_max<T>(en, gl_max<T>(e1, e2));
   }
-  if (!ok || !(en == en) || en > T(3.0e38)) {
+  // the new state must be finite (max/min above drop NaN operands)
+  const T probe = xn[0] + xn[1] + xn[2] + pn[0] + pn[1] + pn[2] + dtinc;
+  if (!ok || !(en == en) || en > T(3.0e38) || !finite_val<T>(probe)) {
     // A non-finite stage is a typed numerical failure (integrator.rs:170-173).
     R.term = kTermFailure;
     R.flags |= kFlagNonFinite;
