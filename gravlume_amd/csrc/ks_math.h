@@ -201,6 +201,59 @@ GL_HD Deriv<T> rhs_from_geometry(const Geom<T>& g, T x, T y, T z, T px, T py, T 
   return d;
 }
 
+// The right-hand side alone, for intermediate DP5 stages that need neither
+// the geometry nor its gradients afterwards. Same equations as geometry() +
+// rhs_from_geometry(), regrouped: with grad r = (x rs, y rs, z G) and
+// grad f = (x Cxy, y Cxy, z Cz),
+//     p' = f q e + (x A, y A, z Az),  A = f q c rs + q^2/2 Cxy,
+//                                     Az = f q c G + q^2/2 Cz,
+// so the gradient vectors are never formed. Returns the geometry's validity.
+template <class T>
+GL_HD bool rhs_only(T x, T y, T z, T px, T py, T pz, T E, T mass, T spin, T spin2, T charge2,
+                    T s, Deriv<T>& d) {
+  const T rho2 = x * x + y * y + z * z;
+  const T b = rho2 - spin2;
+  const T az2 = spin2 * (z * z);
+  const T disc = b * b + T(4) * az2;
+  const T sinv = gl_rsqrt<T>(disc);
+  const T S = disc * sinv;
+  const T r2 = b >= T(0) ? T(0.5) * (b + S) : (T(2) * az2) * gl_rcp<T>(S - b);
+  const T rinv = gl_rsqrt<T>(r2);
+  const T r = r2 * rinv;
+  const T D = r2 + spin2;
+  const T kinv = gl_rcp<T>(D);
+  const T tm = T(2) * mass;
+  const T f = (tm * r - charge2) * sinv;
+  const T rs = r * sinv;
+  const T G = D * rinv * sinv;
+  const T tfs = T(2) * f * sinv;
+  const T cxy = (tm * rs - tfs * b) * sinv;
+  const T cz = (tm * G - tfs * (rho2 + spin2)) * sinv;
+  const T sr = s * r;
+  const T lx = (sr * x + spin * y) * kinv;
+  const T ly = (sr * y - spin * x) * kinv;
+  const T lz = s * z * rinv;
+  const T q = E + (lx * px + ly * py + lz * pz);
+  const T fq = f * q;
+  d.dx = px - fq * lx;
+  d.dy = py - fq * ly;
+  d.dz = pz - fq * lz;
+  d.dt = E + fq;
+  const T ex = (sr * px - spin * py) * kinv;
+  const T ey = (spin * px + sr * py) * kinv;
+  const T ez = s * pz * rinv;
+  const T c = (s * (x * px + y * py) - T(2) * r * (lx * px + ly * py)) * kinv -
+              s * z * pz * (rinv * rinv);
+  const T hq2 = T(0.5) * q * q;
+  const T fqc = fq * c;
+  const T axy = fqc * rs + hq2 * cxy;
+  const T az = fqc * G + hq2 * cz;
+  d.dpx = fq * ex + x * axy;
+  d.dpy = fq * ey + y * axy;
+  d.dpz = fq * ez + z * az;
+  return (r2 > T(0)) && (S > T(0)) && (rho2 < T(1e30));
+}
+
 // Coordinate radial rate dr/dlambda along the Hamilton flow.
 template <class T>
 GL_HD T radial_rate(const Geom<T>& g, const Deriv<T>& d) {

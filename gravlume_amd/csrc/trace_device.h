@@ -696,7 +696,18 @@ GL_HD void step(const SceneT<T>& sc, RayT<T>& R, T omega, bool reproject = false
         }
       }
     }
-    ok = eval<T>(sc, ys[0], ys[1], ys[2], ys[3], ys[4], ys[5], R.E, d, g) && ok;
+    // Intermediate stages need only the derivative; the endpoint (s = 6)
+    // also the geometry. Finiteness of every stage is checked once below,
+    // through the error estimate and the new state.
+    if (s < 6) {
+      ok = rhs_only<T>(ys[0], ys[1], ys[2], ys[3], ys[4], ys[5], R.E, sc.mass, sc.spin, sc.spin2,
+                       sc.charge2, sc.chart, d) && ok;
+    } else {
+      g = geometry<T>(ys[0], ys[1], ys[2], sc.mass, sc.spin, sc.spin2, sc.charge2, sc.chart);
+      d = rhs_from_geometry<T>(g, ys[0], ys[1], ys[2], ys[3], ys[4], ys[5], R.E, sc.spin,
+                               sc.chart);
+      ok = g.ok && ok;
+    }
     K[s][0] = d.dt;
     K[s][1] = d.dx;
     K[s][2] = d.dy;
@@ -727,8 +738,11 @@ GL_HD void step(const SceneT<T>& sc, RayT<T>& R, T omega, bool reproject = false
            gl_rcp<T>(sc.atol + sc.rtol * gl_max<T>(gl_abs<T>(y0[c + 3]), gl_abs<T>(pn[c])));
     en = gl_max<T>(en, gl_max<T>(e1, e2));
   }
-  // the new state must be finite (max/min above drop NaN operands)
-  const T probe = xn[0] + xn[1] + xn[2] + pn[0] + pn[1] + pn[2] + dtinc;
+  // every stage, the error estimate and the new state must be finite (the
+  // max/min above drop NaN operands; a non-finite stage reaches the error or
+  // the new state through its weights or through the later stages)
+  const T probe = xn[0] + xn[1] + xn[2] + pn[0] + pn[1] + pn[2] + dtinc + err[0] + err[1] +
+                  err[2] + err[3] + err[4] + err[5] + err[6];
   if (!ok || !(en == en) || en > T(3.0e38) || !finite_val<T>(probe)) {
     // A non-finite stage is a typed numerical failure (integrator.rs:170-173).
     R.term = kTermFailure;

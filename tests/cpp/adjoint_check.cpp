@@ -1,7 +1,8 @@
 // Host checks of the discrete adjoint (gravlume_amd/csrc/adjoint.h), in double:
 //  1. rhs_vjp against central differences of the forward RHS, for every
 //     input, at seeded points on both branches of the radius root;
-//  2. the frozen-step trajectory adjoint (step_adjoint over N DP5 steps, then
+//  2. rhs_only against geometry + rhs_from_geometry (same derivative);
+//  3. the frozen-step trajectory adjoint (step_adjoint over N DP5 steps, then
 //     terminal_reverse through the dense-output equator crossing with its
 //     implicit fraction theta*) against central differences of the same map
 //     y0 -> y(theta*) with respect to the initial state, E, a and Q^2.
@@ -148,6 +149,27 @@ int main() {
     ++cases;
   }
   std::printf("rhs_vjp cases %d worst %.3e\n", cases, worst);
+  // rhs_only (the regrouped derivative of the intermediate stages) against
+  // geometry + rhs_from_geometry
+  double worst_only = 0.0;
+  for (int t = 0; t < 2000; ++t) {
+    const double s = (t & 1) ? 1.0 : -1.0;
+    const double scale = (t % 4 < 2) ? 12.0 : 0.6;
+    double v[10] = {scale * U(rng), scale * U(rng), scale * U(rng), U(rng), U(rng), U(rng),
+                    1.0 + 0.2 * U(rng), 0.95 * U(rng), 0.15 * (U(rng) + 1.0), 1.0};
+    if (v[0] * v[0] + v[1] * v[1] + v[2] * v[2] < 0.05) continue;
+    Geom<double> g = geometry<double>(v[0], v[1], v[2], 1.0, v[7], v[7] * v[7], v[8], s);
+    Deriv<double> a = rhs_from_geometry<double>(g, v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], s);
+    Deriv<double> b;
+    rhs_only<double>(v[0], v[1], v[2], v[3], v[4], v[5], v[6], 1.0, v[7], v[7] * v[7], v[8], s, b);
+    const double da[7] = {a.dt, a.dx, a.dy, a.dz, a.dpx, a.dpy, a.dpz};
+    const double db[7] = {b.dt, b.dx, b.dy, b.dz, b.dpx, b.dpy, b.dpz};
+    double mag = 1.0;
+    for (double q : da) mag = std::fmax(mag, std::fabs(q));
+    for (int i = 0; i < 7; ++i) worst_only = std::fmax(worst_only, std::fabs(da[i] - db[i]) / mag);
+  }
+  std::printf("rhs_only worst %.3e\n", worst_only);
+  if (worst_only > 1e-12) return 1;
   const double worst_traj = trajectory_check();
   std::printf("trajectory worst %.3e\n", worst_traj);
   return (worst <= 1e-7 && worst_traj <= 1e-6) ? 0 : 1;
