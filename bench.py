@@ -369,10 +369,10 @@ def main():
 
     t_pers, k_pers, f_pers = timed(lambda: pipe.encode_persistent([target]))
     stats = pipe.queue_stats()
-    # host-inclusive rate: each frame's 60 B/ray of output planes copied to
+    # host-inclusive rate: each frame's 68 B/ray of output planes copied to
     # pinned host memory after the trace (PCIe), frames back to back
     planes = ("status", "steps", "rejected", "observable", "travel_time", "affine", "drift",
-              "branch", "radiance")
+              "branch", "radiance", "rgba16f")
     pinned = {k: torch.empty(getattr(target, k).shape, dtype=getattr(target, k).dtype,
                              pin_memory=True) for k in planes}
     torch.cuda.synchronize()
@@ -424,6 +424,11 @@ def main():
         "lane_utilization": stats["attempted_steps"] / max(1, stats["lane_slots"]),
         "persistent_config": tuned["config"],
         "terminations": {int(k): int(v) for k, v in zip(*np.unique(host["termination"], return_counts=True))},
+        "surface_texels": {
+            "radiance": int(np.sum((host["termination"] == 7) & (host["radiance"][:, 0] >= 0))),
+            "not_representable": int(np.sum((host["termination"] == 7) & (host["radiance"][:, 0] < 0))),
+            "model": f"{scene.emission} T_6 = {scene.six_temperature():.6g} K, I_6 = {scene.intensity_at_six}, vacuum",
+        },
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "frac_of_nonpacked_fp32": achieved / 78.6,
                      "traffic": measured_traffic(),

@@ -13,6 +13,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libgravlume_hip.so")
 
 GL_OK = 0
+GL_ERR_INVALID_ARGUMENT = 1
+GL_ERR_DOMAIN = 2
+GL_ERR_NOT_REPRESENTABLE = 3
+GL_ERR_DEVICE = 4
+GL_ERR_CAPACITY = 5
 STATUS_NAMES = {0: "GL_OK", 1: "GL_ERR_INVALID_ARGUMENT", 2: "GL_ERR_DOMAIN",
                 3: "GL_ERR_NOT_REPRESENTABLE", 4: "GL_ERR_DEVICE", 5: "GL_ERR_CAPACITY"}
 
@@ -45,6 +50,15 @@ OPT_F64_HORIZON_STEPS = 2
 OPT_F64_DISK_ROOT = 4
 OPT_ESCAPE_MAP = 8
 
+EMISSION_BOLOMETRIC = 0
+EMISSION_BLACKBODY = 1
+TRANSPORT_VACUUM = 0
+TRANSPORT_SLAB = 1
+
+BLACKBODY_LUT_ENTRIES = 4097
+BLACKBODY_LUT_MIN_LOG2_K = -8.0
+BLACKBODY_LUT_PER_OCTAVE = 128.0
+
 
 class SceneDesc(C.Structure):
     _fields_ = [
@@ -56,7 +70,10 @@ class SceneDesc(C.Structure):
         ("subpixel", C.c_double * 2), ("escape_radius", C.c_double),
         ("horizon_offset", C.c_double), ("has_surface", C.c_int32),
         ("surface_inner", C.c_double), ("surface_outer", C.c_double),
-        ("temperature_inner", C.c_double), ("rtol", C.c_double), ("atol", C.c_double),
+        ("emission_model", C.c_uint32), ("transport_model", C.c_uint32),
+        ("intensity_at_six", C.c_double), ("temperature_at_six", C.c_double),
+        ("optical_depth", C.c_double), ("slab_emission", C.c_double),
+        ("slab_temperature", C.c_double), ("rtol", C.c_double), ("atol", C.c_double),
         ("initial_step", C.c_double), ("minimum_step", C.c_double),
         ("maximum_step", C.c_double), ("maximum_step_per_radius", C.c_double),
         ("max_accepted_steps", C.c_uint32), ("max_consecutive_rejects", C.c_uint32),
@@ -75,7 +92,7 @@ class RayOutputsC(C.Structure):
     _fields_ = [("status", C.c_void_p), ("steps", C.c_void_p), ("rejected", C.c_void_p),
                 ("observable", C.c_void_p), ("travel_time", C.c_void_p), ("affine", C.c_void_p),
                 ("drift", C.c_void_p), ("branch", C.c_void_p), ("radiance", C.c_void_p),
-                ("count", C.c_size_t)]
+                ("rgba16f", C.c_void_p), ("count", C.c_size_t)]
 
 
 WORK_IMAGE = 0
@@ -112,6 +129,8 @@ SIGNATURES = {
     "gl_geodesic_batch_trace": (C.c_int, [_P, C.c_uint32, _P, _P, C.c_size_t,
                                           C.POINTER(RayOutputsC), _P]),
     "gl_scene_validate": (C.c_int, [C.POINTER(SceneDesc)]),
+    "gl_blackbody_log2_fractions": (C.c_int, [C.c_double, C.POINTER(C.c_double)]),
+    "gl_blackbody_lut": (C.c_int, [C.POINTER(C.c_float), C.c_size_t]),
     "gl_trace_encode_work": (C.c_int, [_P, C.POINTER(WorkItem), C.c_size_t, _P]),
     "gl_geodesic_batch_trace_persistent": (C.c_int, [_P, C.c_uint32, _P, _P, C.c_size_t,
                                                      C.POINTER(PersistentConfig),

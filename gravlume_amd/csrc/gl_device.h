@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "../../include/gravlume.h"
 #include "certificate.h"
 #include "gl_types.h"
 #include "trace_device.h"
@@ -117,6 +118,94 @@ __device__ void init_state_ray(const SceneDev& s, const SceneT<float>& sc, const
   R.h = dir * s.h_init;
 }
 
+// ---------------------------------------------------------------------------
+// Surface radiance (f64, once per surface hit)
+
+constexpr double kHalfMax = 65504.0;                        // largest finite binary16
+constexpr double kMinNormal32 = 1.1754943508222875e-38;     // 2^-126
+
+// log2 band fractions of a blackbody at T from the band table, linear in
+// log2 T between entries (blackbody_surface_preview.wgsl:10-33); false
+// outside the table or for a malformed entry.
+__device__ __forceinline__ bool band_log2_fractions(const float* lut, double t, double f[3]) {
+  if (lut == nullptr || !(t > 0.0) || !(t < 1.0e300)) return false;
+  const double c = (::log2(t) - GL_BLACKBODY_LUT_MIN_LOG2_K) * GL_BLACKBODY_LUT_PER_OCTAVE;
+  if (!(c >= 0.0 && c <= double(GL_BLACKBODY_LUT_ENTRIES - 1))) return false;
+  const int i = min(int(c), GL_BLACKBODY_LUT_ENTRIES - 2);
+  const double w = c - double(i);
+  const float4 lo = reinterpret_cast<const float4*>(lut)[i];
+  const float4 hi = reinterpret_cast<const float4*>(lut)[i + 1];
+  f[0] = double(lo.x) + w * (double(hi.x) - double(lo.x));
+  f[1] = double(lo.y) + w * (double(hi.y) - double(lo.y));
+  f[2] = double(lo.z) + w * (double(hi.z) - double(lo.z));
+  return f[0] <= 0.0 && f[1] <= 0.0 && f[2] <= 0.0;
+}
+
+// A positive radiance product resolved for RGBA16F: 0 below the f32 normal
+// range, false above the largest half (surface_transport.wgsl:36-80).
+__device__ __forceinline__ bool resolve_half(double v, double& out) {
+  if (!(v <= kHalfMax)) return false;
+  out = v < kMinNormal32 ? 0.0 : v;
+  return true;
+}
+
+struct Rgb {
+  float r, g, b;
+};
+
+// Observed radiance of a surface hit at source radius r (M = 1) with
+// frequency ratio g: the inverse-cube emitter attenuated by the slab,
+// I = I_6 (r/6)^-3 g^4 exp(-tau), plus the slab's own emission
+// (surface_transport.wgsl:109-223); for a blackbody emitter per band, the
+// incoming term scaled by the band fractions at the observed temperature
+// g T_6 (r/6)^-3/4 and the slab term by those at its temperature
+// (blackbody_surface_preview.wgsl:35-80). (-1, -1, -1) when the result is
+// not representable in RGBA16F.
+__device__ __noinline__ Rgb surface_radiance(const SceneDev& s, float r, float g) {
+  const Rgb bad = {-1.0f, -1.0f, -1.0f};
+  const double rr = double(r) / 6.0, gd = double(g);
+  if (!(rr >= kMinNormal32) || !(gd > 0.0) || !(rr < 1.0e30)) return bad;
+  const double g2 = gd * gd;
+  const double incoming = s.intensity6_d * s.transmittance_d / (rr * rr * rr) * (g2 * g2);
+  const double src = s.slab_emission_d;
+  if (s.emission_model != GL_EMISSION_BLACKBODY) {
+    double v;
+    if (!resolve_half(incoming, v) || !(v + src <= kHalfMax)) return bad;
+    const float o = float(v + src);
+    return Rgb{o, o, o};
+  }
+  double fi[3], fs[3] = {0.0, 0.0, 0.0};
+  const double t_obs = s.temperature6_d / __builtin_sqrt(rr * __builtin_sqrt(rr)) * gd;
+  if (!band_log2_fractions(s.lut, t_obs, fi)) return bad;
+  if (src > 0.0 && !band_log2_fractions(s.lut, s.slab_temperature_d, fs)) return bad;
+  double out[3];
+#pragma unroll
+  for (int b = 0; b < 3; ++b) {
+    double in_b, src_b = 0.0;
+    if (!resolve_half(incoming * ::exp2(fi[b]), in_b)) return bad;
+    if (src > 0.0 && !resolve_half(src * ::exp2(fs[b]), src_b)) return bad;
+    if (!(in_b + src_b <= kHalfMax)) return bad;
+    out[b] = in_b + src_b;
+  }
+  return Rgb{float(out[0]), float(out[1]), float(out[2])};
+}
+
+// Failure colours of non-physical terminals (lensing_preview.wgsl:7-18).
+__device__ __forceinline__ Rgb failure_colour(uint32_t term) {
+  if (term == kTermSingularity) return Rgb{0.0f, 1.0f, 1.0f};
+  if (term == kTermStepExhaustion) return Rgb{1.0f, 0.25f, 0.0f};
+  if (term == kTermUncertain) return Rgb{1.0f, 1.0f, 0.0f};
+  return Rgb{1.0f, 0.0f, 1.0f};
+}
+
+// One RGBA16F texel, round-to-nearest-even per channel, one 8-byte store.
+__device__ __forceinline__ void store_rgba16f(uint16_t* plane, uint32_t idx, Rgb c, float a) {
+  const _Float16 h[4] = {_Float16(c.r), _Float16(c.g), _Float16(c.b), _Float16(a)};
+  uint2 v;
+  __builtin_memcpy(&v, h, 8);
+  reinterpret_cast<uint2*>(plane)[idx] = v;
+}
+
 // Terminal observables from the localized state: the unit traversal
 // direction at an escape (geodesic.rs:480-497), or the source anchor and
 // frequency ratio of the prograde emitter in f64 at a surface hit
@@ -165,32 +254,43 @@ __device__ __noinline__ void write_terminal(const OutPlanes& o, uint32_t idx, co
     o.observable[n + idx] = ob[1];
     o.observable[2 * n + idx] = ob[2];
   }
-  if (o.radiance) {
-    float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
-    if (term == kTermEscape && map.rgb && (s.options & kOptEscapeMap)) {
-      // equirectangular lookup of the escape direction
-      float lon = atan2f(ob[1], ob[0]);
-      float colat = acosf(fminf(fmaxf(ob[2], -1.0f), 1.0f));
-      int iu = int((lon + 3.14159265f) * (0.15915494f * float(map.w)));
-      int iv = int(colat * (0.31830989f * float(map.h)));
-      iu = min(max(iu, 0), int(map.w) - 1);
-      iv = min(max(iv, 0), int(map.h) - 1);
-      size_t t = size_t(iv) * map.w + size_t(iu);
-      size_t plane = size_t(map.w) * map.h;
-      c0 = map.rgb[t];
-      c1 = map.rgb[plane + t];
-      c2 = map.rgb[2 * plane + t];
-    } else if (term == kTermSurface && ob[0] > 0.0f) {
-      float r = ob[0], g = ob[2];
-      float ratio = r / s.disk_in;
-      c0 = g * s.temperature_inner * powf(ratio, -0.75f);  // observed temperature
-      float rr = r / 6.0f;
-      float g2 = g * g;
-      c1 = g2 * g2 / (rr * rr * rr);  // g^4 I_6 (r/6)^-3 with I_6 = 1
+  if (o.radiance || o.rgba16f) {
+    Rgb c = {0.0f, 0.0f, 0.0f};
+    float alpha = -float(term);
+    if (term == kTermEscape) {
+      alpha = 1.0f;
+      if (map.rgb && (s.options & kOptEscapeMap)) {
+        // equirectangular lookup of the escape direction
+        float lon = atan2f(ob[1], ob[0]);
+        float colat = acosf(fminf(fmaxf(ob[2], -1.0f), 1.0f));
+        int iu = int((lon + 3.14159265f) * (0.15915494f * float(map.w)));
+        int iv = int(colat * (0.31830989f * float(map.h)));
+        iu = min(max(iu, 0), int(map.w) - 1);
+        iv = min(max(iv, 0), int(map.h) - 1);
+        size_t t = size_t(iv) * map.w + size_t(iu);
+        size_t plane = size_t(map.w) * map.h;
+        c = Rgb{map.rgb[t], map.rgb[plane + t], map.rgb[2 * plane + t]};
+      }
+    } else if (term == kTermSurface) {
+      c = ob[0] > 0.0f ? surface_radiance(s, ob[0], ob[2]) : Rgb{-1.0f, -1.0f, -1.0f};
+      alpha = 2.0f;
+    } else if (term == kTermHorizon) {
+      alpha = 0.0f;
     }
-    o.radiance[idx] = c0;
-    o.radiance[n + idx] = c1;
-    o.radiance[2 * n + idx] = c2;
+    if (o.radiance) {
+      o.radiance[idx] = c.r;
+      o.radiance[n + idx] = c.g;
+      o.radiance[2 * n + idx] = c.b;
+    }
+    if (o.rgba16f) {
+      if (term == kTermSurface && c.r < 0.0f) {
+        c = failure_colour(kTermFailure);
+        alpha = -float(kTermFailure);
+      } else if (term != kTermEscape && term != kTermSurface && term != kTermHorizon) {
+        c = failure_colour(term);
+      }
+      store_rgba16f(o.rgba16f, idx, c, alpha);
+    }
   }
 }
 

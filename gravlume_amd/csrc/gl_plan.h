@@ -33,6 +33,7 @@ struct gl_trace_plan {
   std::vector<SceneHost> host;
   SceneDev* d_scenes = nullptr;
   float* d_map = nullptr;
+  float* d_lut = nullptr;  // blackbody band table, when a scene needs it
   uint32_t map_w = 0, map_h = 0;
   QueueDev* d_queue = nullptr;
   FrameScene* d_frame = nullptr;

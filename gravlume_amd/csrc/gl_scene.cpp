@@ -120,6 +120,71 @@ int extremality_f64(double mass, double spin, double charge) {
   return d > 0 ? 0 : (d == 0 ? 1 : 2);
 }
 
+namespace {
+
+bool f32_normal_or_zero(double v) {
+  const float f = (float)v;
+  return std::isfinite(f) && (f == 0.0f || std::fabs(f) >= 1.17549435e-38f) && !(v > 0.0 && f == 0.0f);
+}
+
+bool in_lut(double t) {
+  return t >= std::exp2(GL_BLACKBODY_LUT_MIN_LOG2_K) &&
+         t <= std::exp2(GL_BLACKBODY_LUT_MIN_LOG2_K + (GL_BLACKBODY_LUT_ENTRIES - 1) / GL_BLACKBODY_LUT_PER_OCTAVE);
+}
+
+}  // namespace
+
+// The emitter and transport checks of the reference domain and GPU packing:
+// emission.rs:71-145 (radii, non-negative intensity, positive temperature),
+// radiation.rs:160-220 (non-negative depth and emission, positive
+// temperature), scene.rs:58-77 (a blackbody emitter behind an emitting slab
+// needs the slab's spectrum), input.rs:166-320 (temperatures inside the band
+// table at both radii, values that stay normal in f32).
+int check_surface_radiance(const gl_scene_desc& d, std::string* err) {
+  if (d.emission_model != GL_EMISSION_BOLOMETRIC && d.emission_model != GL_EMISSION_BLACKBODY)
+    return set_err(err, "unknown emission model"), GL_ERR_INVALID_ARGUMENT;
+  if (d.transport_model != GL_TRANSPORT_VACUUM && d.transport_model != GL_TRANSPORT_SLAB)
+    return set_err(err, "unknown transport model"), GL_ERR_INVALID_ARGUMENT;
+  if (!std::isfinite(d.intensity_at_six) || d.intensity_at_six < 0.0)
+    return set_err(err, "bolometric intensity must be finite and non-negative"), GL_ERR_INVALID_ARGUMENT;
+  if (!f32_normal_or_zero(d.intensity_at_six))
+    return set_err(err, "surface intensity not representable in f32"), GL_ERR_NOT_REPRESENTABLE;
+  const bool blackbody = d.emission_model == GL_EMISSION_BLACKBODY;
+  if (blackbody) {
+    if (!std::isfinite(d.temperature_at_six) || !(d.temperature_at_six > 0.0))
+      return set_err(err, "blackbody temperature must be finite and positive"), GL_ERR_INVALID_ARGUMENT;
+    for (double r : {d.surface_inner, d.surface_outer}) {
+      const double rr = r / 6.0;
+      const double t = d.temperature_at_six / std::sqrt(rr * std::sqrt(rr));
+      if (!in_lut(t))
+        return set_err(err, "surface temperature at the inner or outer radius lies outside the "
+                            "band table [2^-8, 2^24] K"),
+               GL_ERR_DOMAIN;
+    }
+  }
+  if (d.transport_model == GL_TRANSPORT_SLAB) {
+    if (!std::isfinite(d.optical_depth) || d.optical_depth < 0.0)
+      return set_err(err, "optical depth must be finite and non-negative"), GL_ERR_INVALID_ARGUMENT;
+    if (!std::isfinite(d.slab_emission) || d.slab_emission < 0.0)
+      return set_err(err, "slab emission must be finite and non-negative"), GL_ERR_INVALID_ARGUMENT;
+    if (!std::isfinite(d.slab_temperature) || d.slab_temperature < 0.0)
+      return set_err(err, "slab temperature must be finite and non-negative (0: neutral)"), GL_ERR_INVALID_ARGUMENT;
+    if (blackbody && d.slab_emission > 0.0) {
+      if (d.slab_temperature == 0.0)
+        return set_err(err, "blackbody surface transport with nonzero emission requires a resolved "
+                            "spectrum"),
+               GL_ERR_DOMAIN;
+      if (!in_lut(d.slab_temperature))
+        return set_err(err, "slab emission temperature lies outside the band table [2^-8, 2^24] K"),
+               GL_ERR_DOMAIN;
+    }
+    const double t = std::exp(-d.optical_depth);
+    if (!f32_normal_or_zero(t) || (float)t == 0.0f || !f32_normal_or_zero(d.slab_emission))
+      return set_err(err, "surface transport not representable in f32"), GL_ERR_NOT_REPRESENTABLE;
+  }
+  return GL_OK;
+}
+
 int prepare_scene(const gl_scene_desc& d, SceneDev* out, SceneHost* host, std::string* err) {
   std::memset(out, 0, sizeof(SceneDev));
   // Finite inputs.
@@ -162,8 +227,8 @@ int prepare_scene(const gl_scene_desc& d, SceneDev* out, SceneHost* host, std::s
     // events.rs:73-78 and input.rs:342-381: strictly inside the escape boundary.
     if (!(d.surface_outer < d.escape_radius))
       return set_err(err, "surface outer radius must lie strictly inside the escape radius"), GL_ERR_DOMAIN;
-    if (!(d.temperature_inner > 0.0) || !std::isfinite(d.temperature_inner))
-      return set_err(err, "surface temperature must be positive"), GL_ERR_INVALID_ARGUMENT;
+    const int st = check_surface_radiance(d, err);
+    if (st != GL_OK) return st;
   }
 
   // Horizon and extremality, unchanged by the f32 packing (input.rs:76-90).
@@ -266,7 +331,7 @@ int prepare_scene(const gl_scene_desc& d, SceneDev* out, SceneHost* host, std::s
   s.r_escape = (float)d.escape_radius;
   s.disk_in = d.has_surface ? (float)d.surface_inner : -1.0f;
   s.disk_out = d.has_surface ? (float)d.surface_outer : -1.0f;
-  s.temperature_inner = d.has_surface ? (float)d.temperature_inner : 0.0f;
+  s.emission_model = d.has_surface ? d.emission_model : 0u;
   s.arming_band = (float)d.arming_band;
   s.rtol = (float)d.rtol;
   s.atol = (float)d.atol;
@@ -309,6 +374,13 @@ int prepare_scene(const gl_scene_desc& d, SceneDev* out, SceneHost* host, std::s
   s.r_capture_d = r_plus > 0.0 ? r_plus * (1.0 + d.horizon_offset) : -1.0;
   s.disk_in_d = d.has_surface ? d.surface_inner : -1.0;
   s.disk_out_d = d.has_surface ? d.surface_outer : -1.0;
+  const bool slab = d.has_surface && d.transport_model == GL_TRANSPORT_SLAB;
+  s.intensity6_d = d.has_surface ? d.intensity_at_six : 0.0;
+  s.temperature6_d = d.has_surface && d.emission_model == GL_EMISSION_BLACKBODY ? d.temperature_at_six : 0.0;
+  s.transmittance_d = slab ? std::exp(-d.optical_depth) : 1.0;
+  s.slab_emission_d = slab ? d.slab_emission : 0.0;
+  s.slab_temperature_d = slab ? d.slab_temperature : 0.0;
+  s.lut = nullptr;  // set by the plan when it uploads the band table
   if (!std::isfinite(s.tan_half_fov) || !std::isfinite(s.u_t))
     return set_err(err, "scene not representable in f32"), GL_ERR_NOT_REPRESENTABLE;
 

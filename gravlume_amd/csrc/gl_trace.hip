@@ -388,6 +388,7 @@ OutPlanes to_planes(const gl_ray_outputs& o) {
   p.drift = o.drift;
   p.branch = o.branch;
   p.radiance = o.radiance;
+  p.rgba16f = o.rgba16f;
   p.count = o.count;
   return p;
 }
@@ -537,7 +538,19 @@ bool any_f64_horizon(const gl_trace_plan* p) {
 gl_status check_outputs(const gl_ray_outputs* o, size_t rays) {
   if (!o) return fail(GL_ERR_INVALID_ARGUMENT, "outputs must not be NULL");
   if (o->count < rays) return fail(GL_ERR_INVALID_ARGUMENT, "output planes hold fewer rays than the scene");
+  if (reinterpret_cast<uintptr_t>(o->rgba16f) % 8u != 0u)
+    return fail(GL_ERR_INVALID_ARGUMENT, "the RGBA16F plane must be 8-byte aligned");
   return GL_OK;
+}
+
+// The process-wide band table (gl_spectral.cpp), built on first use.
+const std::vector<float>* blackbody_table() {
+  static const std::vector<float> table = [] {
+    std::vector<float> t(size_t(GL_BLACKBODY_LUT_ENTRIES) * 4);
+    if (gl_blackbody_lut(t.data(), t.size()) != GL_OK) t.clear();
+    return t;
+  }();
+  return table.empty() ? nullptr : &table;
 }
 
 // Launch the finisher over however many rays were deferred (read on device).
@@ -784,6 +797,20 @@ gl_status gl_trace_plan_create(const gl_scene_desc* scenes, size_t n_scenes,
     gl_trace_plan_destroy(p);
     return hip_fail(er, what);
   };
+  bool blackbody = false;
+  for (const auto& s : p->scenes) blackbody = blackbody || s.emission_model == GL_EMISSION_BLACKBODY;
+  if (blackbody) {
+    const std::vector<float>* lut = blackbody_table();
+    if (!lut) {
+      gl_trace_plan_destroy(p);
+      return fail(GL_ERR_NOT_REPRESENTABLE, "blackbody band table could not be built");
+    }
+    const size_t bytes = lut->size() * sizeof(float);
+    if ((e = hipMalloc(&p->d_lut, bytes)) != hipSuccess) return cleanup(e, "hipMalloc band table");
+    if ((e = hipMemcpy(p->d_lut, lut->data(), bytes, hipMemcpyHostToDevice)) != hipSuccess)
+      return cleanup(e, "upload band table");
+    for (auto& s : p->scenes) s.lut = p->d_lut;
+  }
   if ((e = hipMalloc(&p->d_scenes, n_scenes * sizeof(SceneDev))) != hipSuccess) return cleanup(e, "hipMalloc scenes");
   if ((e = hipMemcpy(p->d_scenes, p->scenes.data(), n_scenes * sizeof(SceneDev), hipMemcpyHostToDevice)) != hipSuccess)
     return cleanup(e, "upload scenes");
@@ -811,6 +838,7 @@ gl_status gl_trace_plan_destroy(gl_trace_plan* p) {
   if (!p) return GL_OK;
   if (p->d_scenes) (void)hipFree(p->d_scenes);
   if (p->d_map) (void)hipFree(p->d_map);
+  if (p->d_lut) (void)hipFree(p->d_lut);
   if (p->d_queue) (void)hipFree(p->d_queue);
   if (p->d_frame) (void)hipFree(p->d_frame);
   if (p->h_frame) (void)hipHostFree(p->h_frame);

@@ -7,8 +7,9 @@
 namespace gl {
 
 // One scene as the kernels read it: the f32 image of the validated f64 scene
-// plus the few f64 values the f64 islands need. 256 bytes, read once per
-// ray into registers (the role of TraceUniforms, trace/input.rs:19-35).
+// plus the few f64 values the f64 islands and the surface radiance need.
+// 320 bytes, read once per ray into registers (the role of TraceUniforms,
+// trace/input.rs:19-35).
 struct alignas(16) SceneDev {
   // spacetime
   float mass, spin, spin2, charge2;
@@ -18,7 +19,7 @@ struct alignas(16) SceneDev {
   float r_escape;
   // surface
   float disk_in, disk_out;  // disk_in < 0 disables
-  float temperature_inner;  // T at disk_in
+  uint32_t emission_model;  // 0 bolometric, 1 blackbody
   float arming_band;
   // policy
   float rtol, atol;
@@ -42,9 +43,17 @@ struct alignas(16) SceneDev {
   // f64 copies for the f64 islands
   double spin_d, charge2_d, r_horizon_d;
   double r_capture_d, disk_in_d, disk_out_d;
+  // surface radiance (trace/input.rs:166-320): emitter and slab transport
+  double intensity6_d;     // bolometric intensity at r = 6
+  double temperature6_d;   // blackbody temperature at r = 6 [K]
+  double transmittance_d;  // exp(-optical depth), 1 in vacuum
+  double slab_emission_d;  // integrated slab emission (0 in vacuum)
+  double slab_temperature_d;  // its blackbody temperature [K], 0 neutral
+  const float* lut;        // 4097 x 4 log2 band fractions (blackbody scenes)
+  uint32_t pad_[4];
 };
 
-static_assert(sizeof(SceneDev) == 256, "SceneDev must stay 256 bytes");
+static_assert(sizeof(SceneDev) == 320, "SceneDev must stay 320 bytes");
 
 // Work-queue and statistics block of one persistent launch.
 struct alignas(64) QueueDev {
@@ -67,6 +76,7 @@ struct OutPlanes {
   float* drift;
   uint32_t* branch;
   float* radiance;
+  uint16_t* rgba16f;  // 4 halves per ray
   unsigned long long count;
 };
 

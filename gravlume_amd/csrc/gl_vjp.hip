@@ -403,6 +403,7 @@ gl_status gl_vjp_forward(gl_trace_plan* p, uint32_t scene, double inclination,
   planes.drift = out->drift;
   planes.branch = out->branch;
   planes.radiance = out->radiance;
+  planes.rgba16f = out->rgba16f;
   planes.count = out->count;
   plan_mark(p, 0, hs_);
   hipLaunchKernelGGL(k_vjp_forward, dim3(st->tiles), dim3(64), 0, hs_, p->d_scenes, scene, planes,

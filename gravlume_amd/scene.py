@@ -94,9 +94,55 @@ def isco_prograde(spin: float, charge: float = 0.0) -> float:
     return x
 
 
+@dataclass(frozen=True)
+class Transport:
+    """Transport between the equatorial source and the observer
+    (SurfaceTransport, crates/gravlume-domain/src/scene.rs:40-43): vacuum,
+    or a homogeneous grey slab that maps I_in to
+    I_in exp(-optical_depth) + integrated_emission, the emission term being a
+    blackbody at `temperature` K, or neutral when temperature is None
+    (HomogeneousScalarSlab, radiation.rs:54-220). The constructors below
+    follow the reference's; the values are checked at gl_trace_plan_create."""
+
+    slab: bool = False
+    optical_depth: float = 0.0
+    integrated_emission: float = 0.0
+    temperature: float | None = None
+
+    @staticmethod
+    def vacuum() -> "Transport":
+        return Transport()
+
+    @staticmethod
+    def pure_absorption(optical_depth: float) -> "Transport":
+        return Transport(True, optical_depth, 0.0, None)
+
+    @staticmethod
+    def constant_bolometric(optical_depth: float, source: float) -> "Transport":
+        """Constant neutral source function: emission S (1 - e^-tau)."""
+        return Transport(True, optical_depth, source * -math.expm1(-optical_depth), None)
+
+    @staticmethod
+    def constant_blackbody(optical_depth: float, source: float, temperature: float) -> "Transport":
+        return Transport(True, optical_depth, source * -math.expm1(-optical_depth), temperature)
+
+    @staticmethod
+    def pure_emission_bolometric(emission: float) -> "Transport":
+        return Transport(True, 0.0, emission, None)
+
+    @staticmethod
+    def pure_emission_blackbody(emission: float, temperature: float) -> "Transport":
+        return Transport(True, 0.0, emission, temperature)
+
+
 @dataclass
 class Scene:
-    """One image scene; `to_desc()` packs it for gl_trace_plan_create."""
+    """One image scene; `to_desc()` packs it for gl_trace_plan_create.
+
+    The equatorial emitter radiates I(r) = intensity_at_six (r / 6)^-3
+    (emission.rs:12-145); with emission == "blackbody" as a diluted
+    blackbody at T(r) = T_6 (r / 6)^-3/4, T_6 = temperature_at_six, or, when
+    that is None, the T_6 that puts temperature_inner at the inner radius."""
     spin: float = 0.94
     charge: float = 0.0
     inclination_deg: float = 80.0
@@ -106,7 +152,11 @@ class Scene:
     fov_deg: float = 15.0
     chart: int = OUTGOING
     disk: tuple[float, float] | None = None  # (inner, outer) in M; inner None -> ISCO
-    temperature_inner: float = 1.0e7
+    emission: str = "blackbody"  # "bolometric" or "blackbody"
+    intensity_at_six: float = 1.0
+    temperature_at_six: float | None = None
+    temperature_inner: float = 1.0e7  # K at the inner radius, when temperature_at_six is None
+    transport: Transport = field(default_factory=Transport)
     escape_radius: float = 1000.0
     horizon_offset: float = 1.0e-6
     rtol: float = 1.0e-6
@@ -139,6 +189,15 @@ class Scene:
             inner = isco_prograde(self.spin, self.charge)
         return (inner, outer)
 
+    def six_temperature(self) -> float:
+        """T_6 of the blackbody emitter (0 for a bolometric one)."""
+        if self.emission != "blackbody":
+            return 0.0
+        if self.temperature_at_six is not None:
+            return self.temperature_at_six
+        inner = self.disk_radii()[0]
+        return self.temperature_inner * (inner / 6.0) ** 0.75
+
     def observer_xyz(self) -> tuple[float, float, float]:
         return oblate_to_cartesian(self.spin, self.chart, self.observer_radius,
                                    math.radians(self.inclination_deg), 0.0)
@@ -169,7 +228,17 @@ class Scene:
         if radii is not None:
             d.has_surface = 1
             d.surface_inner, d.surface_outer = radii
-            d.temperature_inner = self.temperature_inner
+            if self.emission not in ("bolometric", "blackbody"):
+                raise ValueError(f"unknown emission model {self.emission!r}")
+            d.emission_model = (_capi.EMISSION_BLACKBODY if self.emission == "blackbody"
+                                else _capi.EMISSION_BOLOMETRIC)
+            d.intensity_at_six = self.intensity_at_six
+            d.temperature_at_six = self.six_temperature()
+            tr = self.transport
+            d.transport_model = _capi.TRANSPORT_SLAB if tr.slab else _capi.TRANSPORT_VACUUM
+            d.optical_depth = tr.optical_depth
+            d.slab_emission = tr.integrated_emission
+            d.slab_temperature = 0.0 if tr.temperature is None else tr.temperature
         d.rtol, d.atol = self.rtol, self.atol
         d.initial_step = self.initial_step * self.observer_radius
         d.minimum_step = self.minimum_step

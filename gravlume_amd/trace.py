@@ -55,11 +55,13 @@ class TraceTarget:
         self.drift = torch.zeros(3 * rays, dtype=torch.float32, device=dev)
         self.branch = torch.zeros(rays, dtype=torch.int32, device=dev)
         self.radiance = torch.zeros(3 * rays, dtype=torch.float32, device=dev)
+        # scene-linear RGBA16F texels, 4 halves per ray (R, G, B, A)
+        self.rgba16f = torch.zeros(4 * rays, dtype=torch.int16, device=dev)
 
     def c_struct(self) -> _capi.RayOutputsC:
         o = _capi.RayOutputsC()
         for name in ("status", "steps", "rejected", "observable", "travel_time", "affine",
-                     "drift", "branch", "radiance"):
+                     "drift", "branch", "radiance", "rgba16f"):
             setattr(o, name, getattr(self, name).data_ptr())
         o.count = self.rays
         return o
@@ -84,6 +86,7 @@ class TraceTarget:
             "azimuth_winding": ((br >> 16) & 255).astype(np.uint8).view(np.int8).astype(np.int32),
             "polar_side": ((br >> 24) & 3).astype(np.int32),
             "radiance": self.radiance.cpu().numpy().reshape(3, n).T.copy(),
+            "rgba16f": self.rgba16f.cpu().numpy().view(np.float16).reshape(n, 4).copy(),
         }
         return out
 

@@ -77,6 +77,18 @@ enum {
   GL_OPT_ESCAPE_MAP = 8u           /* look escape directions up in the map     */
 };
 
+/* Surface emission and transport models (EquatorialEmissionModel,
+ * emission.rs:3-10; SurfaceTransport, scene.rs:40-43). */
+enum { GL_EMISSION_BOLOMETRIC = 0u, GL_EMISSION_BLACKBODY = 1u };
+enum { GL_TRANSPORT_VACUUM = 0u, GL_TRANSPORT_SLAB = 1u };
+
+/* The blackbody band table (spectral_lut.rs:5-10): entry i holds the log2
+ * fractions (red 600-700 nm, green 500-600 nm, blue 400-500 nm, 0) of a
+ * blackbody at T = 2^(GL_BLACKBODY_LUT_MIN_LOG2_K + i / PER_OCTAVE) K. */
+#define GL_BLACKBODY_LUT_ENTRIES 4097
+#define GL_BLACKBODY_LUT_MIN_LOG2_K (-8.0)
+#define GL_BLACKBODY_LUT_PER_OCTAVE 128.0
+
 /* One scene: the content of the reference's validated Observation
  * (crates/gravlume-domain/src/scene.rs:99-143, view.rs:28-69), its event
  * configuration (crates/gravlume-reference/src/events.rs:17-105) and the
@@ -97,7 +109,20 @@ typedef struct gl_scene_desc {
   double horizon_offset;   /* capture event r = r_+ (1 + horizon_offset) */
   int32_t has_surface;     /* equatorial prograde Keplerian emitter */
   double surface_inner, surface_outer;
-  double temperature_inner; /* T at surface_inner [K]; T ~ r^-3/4 */
+  /* Emitter (crates/gravlume-domain/src/emission.rs:12-145): bolometric
+   * intensity I(r) = intensity_at_six (r / 6)^-3; with GL_EMISSION_BLACKBODY
+   * a diluted blackbody at T(r) = temperature_at_six (r / 6)^-3/4 [K]. */
+  uint32_t emission_model;  /* GL_EMISSION_BOLOMETRIC or GL_EMISSION_BLACKBODY */
+  uint32_t transport_model; /* GL_TRANSPORT_VACUUM or GL_TRANSPORT_SLAB */
+  double intensity_at_six;
+  double temperature_at_six;
+  /* Homogeneous slab between source and observer (HomogeneousScalarSlab,
+   * crates/gravlume-domain/src/radiation.rs:54-220):
+   * I_out = I_in exp(-optical_depth) + slab_emission, the emission term a
+   * blackbody at slab_temperature [K] (0: neutral bolometric). */
+  double optical_depth;
+  double slab_emission;
+  double slab_temperature;
   /* adaptive Dormand-Prince 5(4) policy */
   double rtol, atol;
   double initial_step, minimum_step, maximum_step;
@@ -129,7 +154,16 @@ typedef struct gl_tile_region {
  *   drift       (max null residual, max |dL_z|, max |dQ|) relative drifts
  *   branch      bits 0-7 radial turnings, 8-15 equatorial crossings,
  *               16-23 azimuth winding (two's complement), 24-25 polar side
- *   radiance    escape: escape-map RGB; surface: (g T(r) [K], g^4 r^-3, 0) */
+ *   radiance    escape: escape-map RGB (0 without a map); surface: the
+ *               observed radiance (surface_transport.wgsl:209-223,
+ *               blackbody_surface_preview.wgsl:37-80): red, green, blue
+ *               band intensities of a blackbody emitter, or the neutral
+ *               bolometric intensity in all three; -1 when the radiance is
+ *               not representable in RGBA16F; 0 for every other class
+ *   rgba16f     the scene-linear RGBA16F texel, 4 halves per ray in R, G, B,
+ *               A order (lensing_preview.wgsl:64-82): surface (radiance, 2),
+ *               escape (map RGB, 1), horizon (0, 0, 0, 0), anything else a
+ *               failure colour with alpha -termination */
 typedef struct gl_ray_outputs {
   uint32_t* status;
   uint32_t* steps;
@@ -140,10 +174,19 @@ typedef struct gl_ray_outputs {
   float* drift;      /* 3 planes */
   uint32_t* branch;
   float* radiance;   /* 3 planes */
+  uint16_t* rgba16f; /* 4 halves per ray, interleaved */
   size_t count;      /* number of rays the planes hold */
 } gl_ray_outputs;
 
 typedef struct gl_trace_plan gl_trace_plan;
+
+/* Host-side spectral helpers (no device needed). gl_blackbody_log2_fractions
+ * gives log2 of the fraction of sigma T^4 in each visible band at T [K]
+ * (blackbody_band_log2_fractions, spectral_lut.rs:72-88);
+ * gl_blackbody_lut fills the GL_BLACKBODY_LUT_ENTRIES x 4 floats the kernels
+ * interpolate (blackbody_log2_fraction_lut, spectral_lut.rs:40-56). */
+gl_status gl_blackbody_log2_fractions(double temperature_kelvin, double* out_log2);
+gl_status gl_blackbody_lut(float* out, size_t n_floats);
 
 /* The scene checks of gl_trace_plan_create for one scene, on the host only
  * (no device needed): GL_OK or the status and gl_last_error() text the plan

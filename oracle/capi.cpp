@@ -205,6 +205,42 @@ int orc_tangent_g_ordered(double a, double q, double incl, double chart, double 
 
 double orc_isco_prograde(double spin, double charge) { return isco_prograde(spin, charge); }
 
+// Blackbody band fractions and surface transport (radiation.cpp).
+int orc_band_log2_fractions(double temperature, double* out3) {
+  for (int b = 0; b < 3; ++b)
+    if (band_log2_fraction(temperature, b, &out3[b])) return 1;
+  return 0;
+}
+double orc_planck_kernel_integral(double lower, double upper) {
+  return planck_kernel_integral(lower, upper);
+}
+int orc_blackbody_band_intensities(double intensity, double temperature, double* out3) {
+  return blackbody_band_intensities(intensity, temperature, out3);
+}
+int orc_transport_bolometric(double incoming, const Transport* tr, double* out3) {
+  return transport_bolometric(incoming, *tr, &out3[0], &out3[1], &out3[2]);
+}
+double orc_constant_source_emission(double optical_depth, double source) {
+  return constant_source_emission(optical_depth, source);
+}
+// surface_radiance for n (r, g) pairs; out holds 9 doubles per pair:
+// emitted, vacuum, observed intensity, optical depth, emitted and observed
+// temperature, three bands.
+int orc_surface_radiance(const Emitter* em, const Transport* tr, const double* r, const double* g,
+                         size_t n, double* out, int* status) {
+  for (size_t i = 0; i < n; ++i) {
+    SurfaceRadiance s{};
+    status[i] = surface_radiance(*em, *tr, r[i], g[i], &s);
+    const double v[9] = {s.emitted_intensity, s.vacuum_intensity, s.observed_intensity,
+                         s.optical_depth, s.emitted_temperature, s.observed_temperature,
+                         s.bands[0], s.bands[1], s.bands[2]};
+    for (int k = 0; k < 9; ++k) out[9 * i + k] = status[i] ? 0.0 : v[k];
+  }
+  return 0;
+}
+size_t orc_sizeof_transport() { return sizeof(Transport); }
+size_t orc_sizeof_emitter() { return sizeof(Emitter); }
+
 int orc_compare_extremality(double m, double a, double q) { return compare_extremality(m, a, q); }
 
 size_t orc_sizeof_outcome() { return sizeof(Outcome); }

@@ -85,6 +85,7 @@ enum SurfaceError : int {
   kSurfEmittedIntensity = 8,
   kSurfObservedIntensity = 9,
   kSurfTemperature = 10,
+  kSurfSpectrum = 11,
 };
 
 struct Spacetime {
@@ -121,6 +122,25 @@ struct Emitter {
   double intensity_at_six = 1.0;
   int blackbody = 0;
   double temperature_at_six = 0.0;
+};
+
+// SurfaceTransport (crates/gravlume-domain/src/scene.rs:40-43) with the
+// HomogeneousScalarSlab (radiation.rs:54-220): slab = 0 is vacuum; otherwise
+// I_out = I_in exp(-optical_depth) + integrated_emission, whose spectrum is a
+// blackbody at `temperature` when blackbody = 1.
+struct Transport {
+  int slab = 0;
+  double optical_depth = 0.0;
+  double integrated_emission = 0.0;
+  int blackbody = 0;
+  double temperature = 0.0;
+};
+
+// The radiance fields of SurfaceObservable (surface.rs:235-290).
+struct SurfaceRadiance {
+  double emitted_intensity, vacuum_intensity, observed_intensity, optical_depth;
+  double emitted_temperature, observed_temperature;
+  double bands[3];  // red, green, blue (blackbody emitters only)
 };
 
 struct Outcome {
@@ -209,6 +229,19 @@ int observable_at(const Spacetime& st, const Emitter& em, double inner, double o
                   const S8& state, double observer_frequency, Outcome* out);
 
 double isco_prograde(double spin, double charge);
+
+// radiation.cpp: log2 of the blackbody fraction in band 0/1/2 (red, green,
+// blue boxcars) at temperature T; 0 on success.
+int band_log2_fraction(double temperature, int band, double* out);
+double planck_kernel_integral(double lower, double upper);
+int blackbody_band_intensities(double intensity, double temperature, double out[3]);
+int transport_bolometric(double incoming, const Transport& tr, double* out, double* tau,
+                         double* transmittance);
+int transport_bands(const double in[3], const Transport& tr, bool blackbody_emitter,
+                    double out[3]);
+double constant_source_emission(double optical_depth, double source);
+int surface_radiance(const Emitter& em, const Transport& tr, double r, double g,
+                     SurfaceRadiance* out);
 
 // Forward-mode tangents (tangent.cpp): g and dg/d(a, Q, inclination, s_x, s_y)
 // of the backward ray through sight-plane point (sx, sy) of the observation

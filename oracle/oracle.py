@@ -52,6 +52,12 @@ class Emitter(C.Structure):
                 ("temperature_at_six", C.c_double)]
 
 
+class Transport(C.Structure):
+    _fields_ = [("slab", C.c_int), ("optical_depth", C.c_double),
+                ("integrated_emission", C.c_double), ("blackbody", C.c_int),
+                ("temperature", C.c_double)]
+
+
 class Outcome(C.Structure):
     _fields_ = [
         ("termination", C.c_int), ("failure", C.c_int), ("candidates", C.c_uint32),
@@ -127,8 +133,18 @@ def lib() -> C.CDLL:
         L.orc_tangent_g_ordered.argtypes = ([C.c_double] * 5 + [C.c_uint32, C.c_uint32,
                                                                 C.c_double, dp, C.c_size_t]
                                             + [C.c_double] * 4 + [C.c_uint, dp, dp, P(C.c_int)])
+        L.orc_band_log2_fractions.argtypes = [C.c_double, dp]
+        L.orc_planck_kernel_integral.argtypes = [C.c_double, C.c_double]
+        L.orc_planck_kernel_integral.restype = C.c_double
+        L.orc_blackbody_band_intensities.argtypes = [C.c_double, C.c_double, dp]
+        L.orc_transport_bolometric.argtypes = [C.c_double, P(Transport), dp]
+        L.orc_constant_source_emission.argtypes = [C.c_double, C.c_double]
+        L.orc_constant_source_emission.restype = C.c_double
+        L.orc_surface_radiance.argtypes = [P(Emitter), P(Transport), dp, dp, C.c_size_t, dp,
+                                           P(C.c_int)]
         for name, struct in (("outcome", Outcome), ("observation", Observation),
-                             ("policy", Policy), ("spacetime", Spacetime), ("events", Events)):
+                             ("policy", Policy), ("spacetime", Spacetime), ("events", Events),
+                             ("transport", Transport), ("emitter", Emitter)):
             fn = getattr(L, f"orc_sizeof_{name}")
             fn.restype = C.c_size_t
             assert fn() == C.sizeof(struct), f"oracle struct layout mismatch: {name}"
@@ -272,6 +288,63 @@ def observable_at(st, emitter, inner, outer, state, omega=1.0):
     rc = lib().orc_observable_at(C.byref(st), C.byref(emitter), inner, outer, _dp(y), omega,
                                  C.byref(out))
     return rc, out
+
+
+def band_log2_fractions(temperature: float):
+    """log2 of the red/green/blue boxcar fractions of a blackbody
+    (radiation.rs:83-101), or None when not representable."""
+    out = np.zeros(3)
+    if lib().orc_band_log2_fractions(float(temperature), _dp(out)):
+        return None
+    return out
+
+
+def planck_kernel_integral(lower: float, upper: float) -> float:
+    return lib().orc_planck_kernel_integral(lower, upper)
+
+
+def blackbody_band_intensities(intensity: float, temperature: float):
+    out = np.zeros(3)
+    if lib().orc_blackbody_band_intensities(intensity, temperature, _dp(out)):
+        return None
+    return out
+
+
+def vacuum() -> Transport:
+    return Transport(0, 0.0, 0.0, 0, 0.0)
+
+
+def slab(optical_depth: float, integrated_emission: float = 0.0,
+         temperature: float | None = None) -> Transport:
+    """HomogeneousScalarSlab with its integrated emission term (neutral when
+    temperature is None)."""
+    return Transport(1, optical_depth, integrated_emission, 0 if temperature is None else 1,
+                     0.0 if temperature is None else temperature)
+
+
+def constant_source_emission(optical_depth: float, source: float) -> float:
+    return lib().orc_constant_source_emission(optical_depth, source)
+
+
+def transport_bolometric(incoming: float, transport: Transport):
+    out = np.zeros(3)
+    if lib().orc_transport_bolometric(incoming, C.byref(transport), _dp(out)):
+        return None
+    return out  # (outgoing, optical depth, transmittance)
+
+
+def surface_radiance(emitter: Emitter, transport: Transport, r, g):
+    """The radiance chain of observable_at (surface.rs:12-60) for arrays of
+    source radius r and frequency ratio g: dict of arrays plus status."""
+    r = np.ascontiguousarray(r, dtype=np.float64).ravel()
+    g = np.ascontiguousarray(g, dtype=np.float64).ravel()
+    out = np.zeros((r.size, 9))
+    status = np.zeros(r.size, dtype=np.int32)
+    lib().orc_surface_radiance(C.byref(emitter), C.byref(transport), _dp(r), _dp(g), r.size,
+                               _dp(out), status.ctypes.data_as(C.POINTER(C.c_int)))
+    return dict(emitted=out[:, 0], vacuum=out[:, 1], observed=out[:, 2], optical_depth=out[:, 3],
+                emitted_temperature=out[:, 4], observed_temperature=out[:, 5],
+                bands=out[:, 6:9], status=status)
 
 
 def rhs(st, state):
