@@ -44,6 +44,7 @@ FLAG_DRIFT_BUDGET = 8
 FLAG_TURNING_ORDER = 16
 FLAG_TIED_EVENTS = 32
 FLAG_CERTIFIED_CAPTURE = 64
+FLAG_RECONSTRUCTED = 128
 
 OPT_CAPTURE_CERTIFICATE = 1
 OPT_F64_HORIZON_STEPS = 2
@@ -132,6 +133,9 @@ SIGNATURES = {
     "gl_blackbody_log2_fractions": (C.c_int, [C.c_double, C.POINTER(C.c_double)]),
     "gl_blackbody_lut": (C.c_int, [C.POINTER(C.c_float), C.c_size_t]),
     "gl_trace_encode_work": (C.c_int, [_P, C.POINTER(WorkItem), C.c_size_t, _P]),
+    "gl_trace_encode_accelerated": (C.c_int, [_P, C.c_uint32, C.POINTER(RayOutputsC), _P]),
+    "gl_trace_refine_shadow": (C.c_int, [_P, C.c_uint32, C.POINTER(RayOutputsC), _P]),
+    "gl_trace_plan_shadow_edges": (C.c_int, [_P, C.POINTER(C.c_uint64), _P]),
     "gl_geodesic_batch_trace_persistent": (C.c_int, [_P, C.c_uint32, _P, _P, C.c_size_t,
                                                      C.POINTER(PersistentConfig),
                                                      C.POINTER(RayOutputsC), _P]),

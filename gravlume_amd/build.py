@@ -12,7 +12,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libgravlume_hip.so")
-SOURCES = ["gl_trace.hip", "gl_scene.cpp", "gl_spectral.cpp", "gl_flops.cpp", "gl_vjp.hip"]
+SOURCES = ["gl_trace.hip", "gl_accel.hip", "gl_scene.cpp", "gl_spectral.cpp", "gl_flops.cpp",
+           "gl_vjp.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 # Packed FP32 instructions are disabled for every HIP translation unit.

@@ -49,10 +49,11 @@ struct Momentum {
   float px, py, pz, E;
 };
 
-__device__ __noinline__ Momentum image_momentum(const SceneDev& s, uint32_t px, uint32_t py) {
+__device__ __noinline__ Momentum image_momentum(const SceneDev& s, uint32_t px, uint32_t py,
+                                                float ox, float oy) {
   const SceneT<float> sc = scene_f32(s);
-  float nx = 2.0f * (float(px) + s.sub_x) / float(s.width) - 1.0f;
-  float ny = 1.0f - 2.0f * (float(py) + s.sub_y) / float(s.height);
+  float nx = 2.0f * (float(px) + ox) / float(s.width) - 1.0f;
+  float ny = 1.0f - 2.0f * (float(py) + oy) / float(s.height);
   float sx = s.aspect * s.tan_half_fov * nx;
   float sy = s.tan_half_fov * ny;
   float inv = 1.0f / sqrtf(1.0f + sx * sx + sy * sy);
@@ -94,9 +95,12 @@ __device__ __noinline__ bool certify_capture(const SceneDev& s, float px, float 
                            s.obs_x * s.obs_x + s.obs_y * s.obs_y, dr_future, E, L, Q);
 }
 
-__device__ __forceinline__ void init_image_ray(const SceneDev& s, const SceneT<float>& sc,
-                                               uint32_t px, uint32_t py, RayT<float>& R) {
-  Momentum m = image_momentum(s, px, py);
+// The backward ray through point (px + ox, py + oy) of the viewport, with
+// the capture certificate applied when the scene enables it.
+__device__ __forceinline__ void init_image_ray_at(const SceneDev& s, const SceneT<float>& sc,
+                                                  uint32_t px, uint32_t py, float ox, float oy,
+                                                  RayT<float>& R) {
+  Momentum m = image_momentum(s, px, py, ox, oy);
   init_ray<float>(sc, R, s.obs_x, s.obs_y, s.obs_z, 0.0f, 0.0f, 0.0f, m.px, m.py, m.pz, m.E,
                   -1.0f, s.polar_side);
   R.h = -s.h_init;
@@ -106,6 +110,12 @@ __device__ __forceinline__ void init_image_ray(const SceneDev& s, const SceneT<f
     R.candidates = 2u;
     R.flags |= kFlagCertified;
   }
+}
+
+// The same at the scene's sample offset.
+__device__ __forceinline__ void init_image_ray(const SceneDev& s, const SceneT<float>& sc,
+                                               uint32_t px, uint32_t py, RayT<float>& R) {
+  init_image_ray_at(s, sc, px, py, s.sub_x, s.sub_y, R);
 }
 
 __device__ void init_state_ray(const SceneDev& s, const SceneT<float>& sc, const double* y,
@@ -190,6 +200,40 @@ __device__ __noinline__ Rgb surface_radiance(const SceneDev& s, float r, float g
   return Rgb{float(out[0]), float(out[1]), float(out[2])};
 }
 
+// Unit traversal direction dx/dlambda (times the trace direction) at an
+// escape state (geodesic.rs:480-497).
+struct Dir3 {
+  float x, y, z;
+  bool ok;
+};
+
+__device__ __forceinline__ Dir3 escape_direction(const SceneDev& s, float dir, float x, float y,
+                                                 float z, float px, float py, float pz, float E) {
+  const SceneT<float> sc = scene_f32(s);
+  Deriv<float> d;
+  Geom<float> g;
+  if (!eval<float>(sc, x, y, z, px, py, pz, E, d, g)) return Dir3{0.0f, 0.0f, 0.0f, false};
+  const float vx = dir * d.dx, vy = dir * d.dy, vz = dir * d.dz;
+  const float inv = 1.0f / sqrtf(vx * vx + vy * vy + vz * vz);
+  return Dir3{vx * inv, vy * inv, vz * inv, true};
+}
+
+// Escape-map texel of a unit escape direction (equirectangular, nearest
+// texel); black without a bound map.
+__device__ __forceinline__ Rgb escape_texel(const MapView& map, const SceneDev& s, float dx, float dy,
+                                            float dz) {
+  if (!map.rgb || !(s.options & kOptEscapeMap)) return Rgb{0.0f, 0.0f, 0.0f};
+  float lon = atan2f(dy, dx);
+  float colat = acosf(fminf(fmaxf(dz, -1.0f), 1.0f));
+  int iu = int((lon + 3.14159265f) * (0.15915494f * float(map.w)));
+  int iv = int(colat * (0.31830989f * float(map.h)));
+  iu = min(max(iu, 0), int(map.w) - 1);
+  iv = min(max(iv, 0), int(map.h) - 1);
+  size_t t = size_t(iv) * map.w + size_t(iu);
+  size_t plane = size_t(map.w) * map.h;
+  return Rgb{map.rgb[t], map.rgb[plane + t], map.rgb[2 * plane + t]};
+}
+
 // Failure colours of non-physical terminals (lensing_preview.wgsl:7-18).
 __device__ __forceinline__ Rgb failure_colour(uint32_t term) {
   if (term == kTermSingularity) return Rgb{0.0f, 1.0f, 1.0f};
@@ -221,18 +265,11 @@ __device__ __noinline__ void write_terminal(const OutPlanes& o, uint32_t idx, co
   const unsigned long long n = o.count;
   float ob[3] = {0.0f, 0.0f, 0.0f};
   if (term == kTermEscape) {
-    const SceneT<float> sc = scene_f32(s);
-    Deriv<float> d;
-    Geom<float> g;
-    if (eval<float>(sc, x, y, z, px, py, pz, E, d, g)) {
-      float vx = dir * d.dx, vy = dir * d.dy, vz = dir * d.dz;
-      float inv = 1.0f / sqrtf(vx * vx + vy * vy + vz * vz);
-      ob[0] = vx * inv;
-      ob[1] = vy * inv;
-      ob[2] = vz * inv;
-    } else {
-      flags |= kFlagNonFinite;
-    }
+    const Dir3 v = escape_direction(s, dir, x, y, z, px, py, pz, E);
+    ob[0] = v.x;
+    ob[1] = v.y;
+    ob[2] = v.z;
+    if (!v.ok) flags |= kFlagNonFinite;
   } else if (term == kTermSurface) {
     double xd = double(x) - double(cx), yd = double(y) - double(cy), zd = double(z) - double(cz);
     Geom<double> g = geometry<double>(xd, yd, zd, double(s.mass), s.spin_d, s.spin_d * s.spin_d,
@@ -259,18 +296,7 @@ __device__ __noinline__ void write_terminal(const OutPlanes& o, uint32_t idx, co
     float alpha = -float(term);
     if (term == kTermEscape) {
       alpha = 1.0f;
-      if (map.rgb && (s.options & kOptEscapeMap)) {
-        // equirectangular lookup of the escape direction
-        float lon = atan2f(ob[1], ob[0]);
-        float colat = acosf(fminf(fmaxf(ob[2], -1.0f), 1.0f));
-        int iu = int((lon + 3.14159265f) * (0.15915494f * float(map.w)));
-        int iv = int(colat * (0.31830989f * float(map.h)));
-        iu = min(max(iu, 0), int(map.w) - 1);
-        iv = min(max(iv, 0), int(map.h) - 1);
-        size_t t = size_t(iv) * map.w + size_t(iu);
-        size_t plane = size_t(map.w) * map.h;
-        c = Rgb{map.rgb[t], map.rgb[plane + t], map.rgb[2 * plane + t]};
-      }
+      c = escape_texel(map, s, ob[0], ob[1], ob[2]);
     } else if (term == kTermSurface) {
       c = ob[0] > 0.0f ? surface_radiance(s, ob[0], ob[2]) : Rgb{-1.0f, -1.0f, -1.0f};
       alpha = 2.0f;

@@ -63,6 +63,12 @@ struct gl_trace_plan {
   int timed_cur = -1;
   double main_sum = 0.0, finish_sum = 0.0;
   uint64_t main_n = 0, finish_n = 0;
+  // Escape-direction stencil nodes and the shadow-edge pixel list of the
+  // accelerated sky trace and coverage refinement (gl_accel.hip).
+  float4* d_nodes = nullptr;
+  size_t node_capacity = 0;
+  uint32_t* d_edges = nullptr;  // [0] count, [1..] pixel indices
+  size_t edge_capacity = 0;
 };
 
 
@@ -70,4 +76,16 @@ namespace gl {
 // Records timing event `which` (0/1 main kernel start/stop, 2/3 finisher)
 // of the current launch on `st` when the plan's timing is enabled.
 void plan_mark(gl_trace_plan* p, int which, hipStream_t st);
+
+// Host helpers of the trace entry points (gl_trace.hip), shared with the
+// accelerated sky trace (gl_accel.hip).
+gl_status hip_fail(hipError_t e, const char* what);
+gl_status check_outputs(const gl_ray_outputs* o, size_t rays);
+FrameScene make_frame(uint32_t scene, uint32_t first, uint32_t rays, const gl_ray_outputs& o);
+gl_status upload_frame(gl_trace_plan* p, const std::vector<FrameScene>& table, hipStream_t st);
+gl_status ensure_defer(gl_trace_plan* p, size_t rays);
+gl_status launch_finisher(gl_trace_plan* p, hipStream_t st);
+MapView map_view(const gl_trace_plan* p);
+DeferBuf defer_buf(const gl_trace_plan* p);
+__global__ void k_queue_reset(QueueDev* q, unsigned long long total, unsigned int* defer_count);
 }  // namespace gl

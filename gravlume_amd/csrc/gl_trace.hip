@@ -371,7 +371,7 @@ gl_status fail(gl_status st, const std::string& msg) {
 }
 }  // namespace gl
 
-namespace {
+namespace gl {
 
 gl_status hip_fail(hipError_t e, const char* what) {
   return fail(GL_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
@@ -393,7 +393,7 @@ OutPlanes to_planes(const gl_ray_outputs& o) {
   return p;
 }
 
-}  // namespace
+}  // namespace gl
 
 struct gl_frame_graph {
   hipGraph_t graph = nullptr;
@@ -401,7 +401,7 @@ struct gl_frame_graph {
   FrameScene* d_table = nullptr;
 };
 
-namespace {
+namespace gl {
 
 gl_status ensure_frame(gl_trace_plan* p, size_t n) {
   if (n <= p->frame_capacity) return GL_OK;
@@ -483,7 +483,7 @@ gl_status harvest(gl_trace_plan* p, gl_trace_plan::TimedLaunch& t) {
   return GL_OK;
 }
 
-}  // namespace
+}  // namespace gl
 
 // Event 0 opens a new timed launch (main kernel start), 1 closes the main
 // kernel, 2 and 3 bracket the f64 finisher of the same launch.
@@ -509,7 +509,7 @@ void gl::plan_mark(gl_trace_plan* p, int which, hipStream_t st) {
   if (which == 3) t.finisher = true;
 }
 
-namespace {
+namespace gl {
 
 void mark(gl_trace_plan* p, int which, hipStream_t st) { gl::plan_mark(p, which, st); }
 
@@ -747,7 +747,7 @@ gl_status persistent_frame(gl_trace_plan* p, const uint32_t* scenes, size_t n_sc
   return launch_persistent(p, p->d_frame, table.size(), total, rays, st, p->pcfg);
 }
 
-}  // namespace
+}  // namespace gl
 
 extern "C" {
 
@@ -839,6 +839,8 @@ gl_status gl_trace_plan_destroy(gl_trace_plan* p) {
   if (p->d_scenes) (void)hipFree(p->d_scenes);
   if (p->d_map) (void)hipFree(p->d_map);
   if (p->d_lut) (void)hipFree(p->d_lut);
+  if (p->d_nodes) (void)hipFree(p->d_nodes);
+  if (p->d_edges) (void)hipFree(p->d_edges);
   if (p->d_queue) (void)hipFree(p->d_queue);
   if (p->d_frame) (void)hipFree(p->d_frame);
   if (p->h_frame) (void)hipHostFree(p->h_frame);

@@ -199,6 +199,7 @@ enum : uint32_t {
   kFlagTurningOrder = 16u,
   kFlagTied = 32u,
   kFlagCertified = 64u,  // horizon proven by the interval capture certificate
+  kFlagReconstructed = 128u,  // escape direction interpolated from the stencil
 };
 enum : uint32_t {
   kOptCertificate = 1u,

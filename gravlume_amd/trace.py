@@ -172,6 +172,31 @@ class TracePipeline:
         _capi.check(self._lib.gl_trace_encode_persistent(self._handle, ids, n, outs,
                                                          _stream_handle(stream)))
 
+    def encode_accelerated(self, target: TraceTarget, stream=None) -> None:
+        """Accelerated sky trace of the target's scene (no surface): the
+        escape-direction stencil fills stable 8x8 tiles from their traced
+        4-pixel nodes, the rest is traced in full
+        (geodesic_acceleration.wgsl:330-601)."""
+        k = target.scene if target.scene is not None else 0
+        out = target.c_struct()
+        _capi.check(self._lib.gl_trace_encode_accelerated(self._handle, k, C.byref(out),
+                                                          _stream_handle(stream)))
+
+    def refine_shadow(self, target: TraceTarget, stream=None) -> None:
+        """Four-sample coverage of the traced image's horizon/escape edge
+        pixels into the RGBA16F plane (shadow_coverage.wgsl:1-136)."""
+        k = target.scene if target.scene is not None else 0
+        out = target.c_struct()
+        _capi.check(self._lib.gl_trace_refine_shadow(self._handle, k, C.byref(out),
+                                                     _stream_handle(stream)))
+
+    def shadow_edges(self, stream=None) -> int:
+        """Edge pixels listed by the last refine_shadow (synchronizes)."""
+        n = C.c_uint64()
+        _capi.check(self._lib.gl_trace_plan_shadow_edges(self._handle, C.byref(n),
+                                                         _stream_handle(stream)))
+        return n.value
+
     def set_persistent_config(self, block_size=256, blocks_per_cu=2, refill_interval=8,
                               refill_threshold=16, waves_per_simd=2) -> None:
         c = _capi.PersistentConfig(block_size, blocks_per_cu, refill_interval, refill_threshold,

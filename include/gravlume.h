@@ -62,8 +62,11 @@ enum {
   GL_FLAG_DRIFT_BUDGET = 8u,     /* invariant drift above the scene limit    */
   GL_FLAG_TURNING_ORDER = 16u,   /* turning bracket overlaps the terminal    */
   GL_FLAG_TIED_EVENTS = 32u,     /* several event candidates within the tie  */
-  GL_FLAG_CERTIFIED_CAPTURE = 64u /* horizon class proven by the interval
-                                     capture certificate, not integrated   */
+  GL_FLAG_CERTIFIED_CAPTURE = 64u, /* horizon class proven by the interval
+                                      capture certificate, not integrated  */
+  GL_FLAG_RECONSTRUCTED = 128u      /* escape direction interpolated from the
+                                      escape-direction stencil, not traced:
+                                      steps, times, drifts and branch key 0 */
 };
 
 /* Option bits of gl_scene_desc.options. */
@@ -230,6 +233,27 @@ typedef struct gl_work_item {
 } gl_work_item;
 gl_status gl_trace_encode_work(gl_trace_plan* plan, const gl_work_item* items, size_t n_items,
                                void* stream);
+
+/* Accelerated sky trace (trace_scene_accelerated and trace_escape_map_nodes,
+ * geodesic_acceleration.wgsl:330-601) of one scene without a surface: the
+ * pixels (4i, 4j) are traced in full, then every 8x8 tile whose 3x3 such
+ * nodes all escape and agree with the bilinear blend of its corners within
+ * min(half a pixel, 3e-4 rad) gets the blended escape direction for its
+ * other pixels (GL_FLAG_RECONSTRUCTED); all other pixels are traced in full.
+ * Needs the observable plane. */
+gl_status gl_trace_encode_accelerated(gl_trace_plan* plan, uint32_t scene,
+                                      const gl_ray_outputs* out, void* stream);
+
+/* Shadow-edge coverage (classify_shadow_edges and refine_shadow_edges,
+ * shadow_coverage.wgsl:1-136) on a traced image of `scene`: every pixel of
+ * class horizon or escape with a 4-neighbour of the other class is re-traced
+ * at four rotated-grid sub-samples (f32); its RGBA16F texel becomes the mean
+ * escape colour with alpha = escaping fraction, unless a sub-sample ends in
+ * another class. Reads the status plane, rewrites RGBA16F texels only. */
+gl_status gl_trace_refine_shadow(gl_trace_plan* plan, uint32_t scene, const gl_ray_outputs* out,
+                                 void* stream);
+/* Edge pixels listed by the last gl_trace_refine_shadow (synchronizes the stream). */
+gl_status gl_trace_plan_shadow_edges(gl_trace_plan* plan, uint64_t* count, void* stream);
 
 /* Launch parameters of the persistent kernel (set by the autotuner). */
 typedef struct gl_persistent_config {
