@@ -330,7 +330,7 @@ def accelerated_sky_leg(g, S, torch, repeats=5):
                  capture_certificate=True)
     pipe = g.TracePipeline([sc])
     pipe.enable_timing(True)
-    full, acc = pipe.create_target(0), pipe.create_target(0)
+    full, full_p, acc = pipe.create_target(0), pipe.create_target(0), pipe.create_target(0)
 
     def run(fn):
         fn()  # warm-up
@@ -349,9 +349,9 @@ def accelerated_sky_leg(g, S, torch, repeats=5):
                        "60deg, FOV 45deg, escape r=200M into the map, capture certificate, f32 DP5 "
                        "rtol 1e-6"}
     res["full_plain"] = run(lambda: pipe.encode_batch(full))
-    res["full_persistent"] = run(lambda: pipe.encode_persistent([full]))
+    res["full_persistent"] = run(lambda: pipe.encode_persistent([full_p]))
     res["accelerated"] = run(lambda: pipe.encode_accelerated(acc))
-    hf, ha = full.to_host(), acc.to_host()
+    hf, ha = full.to_host(), acc.to_host()  # against the plain full trace
     rec = (ha["flags"] & 128) != 0
     esc = hf["termination"] == 2
     chord = np.linalg.norm(ha["observable"][esc].astype(np.float64) - hf["observable"][esc], axis=1)

@@ -136,6 +136,7 @@ SIGNATURES = {
     "gl_trace_encode_accelerated": (C.c_int, [_P, C.c_uint32, C.POINTER(RayOutputsC), _P]),
     "gl_trace_refine_shadow": (C.c_int, [_P, C.c_uint32, C.POINTER(RayOutputsC), _P]),
     "gl_trace_plan_shadow_edges": (C.c_int, [_P, C.POINTER(C.c_uint64), _P]),
+    "gl_trace_surface_footprint": (C.c_int, [_P, C.c_uint32, _P, C.c_size_t, _P, _P, _P]),
     "gl_geodesic_batch_trace_persistent": (C.c_int, [_P, C.c_uint32, _P, _P, C.c_size_t,
                                                      C.POINTER(PersistentConfig),
                                                      C.POINTER(RayOutputsC), _P]),

@@ -218,6 +218,28 @@ __device__ __forceinline__ Dir3 escape_direction(const SceneDev& s, float dir, f
   return Dir3{vx * inv, vy * inv, vz * inv, true};
 }
 
+// Source anchor (r, phi_s) and frequency ratio g of a surface hit, in f64
+// from the localized f32 state and its Kahan compensation
+// (surface.rs:75-146).
+struct Anchor {
+  float r, phi, g;
+  bool ok;
+};
+
+__device__ __forceinline__ Anchor surface_anchor(const SceneDev& s, float x, float y, float z,
+                                                 float cx, float cy, float cz, float px, float py,
+                                                 float E) {
+  const double xd = double(x) - double(cx), yd = double(y) - double(cy), zd = double(z) - double(cz);
+  const Geom<double> g = geometry<double>(xd, yd, zd, double(s.mass), s.spin_d,
+                                          s.spin_d * s.spin_d, s.charge2_d, double(s.chart));
+  const double lz = xd * double(py) - yd * double(px);
+  double o3[3];
+  if (g.ok && surface_observable<double>(double(s.mass), s.spin_d, s.charge2_d, double(s.chart),
+                                         double(s.omega), xd, yd, g.r, double(E), lz, o3))
+    return Anchor{float(o3[0]), float(o3[1]), float(o3[2]), true};
+  return Anchor{0.0f, 0.0f, 0.0f, false};
+}
+
 // Escape-map texel of a unit escape direction (equirectangular, nearest
 // texel); black without a bound map.
 __device__ __forceinline__ Rgb escape_texel(const MapView& map, const SceneDev& s, float dx, float dy,
@@ -271,19 +293,11 @@ __device__ __noinline__ void write_terminal(const OutPlanes& o, uint32_t idx, co
     ob[2] = v.z;
     if (!v.ok) flags |= kFlagNonFinite;
   } else if (term == kTermSurface) {
-    double xd = double(x) - double(cx), yd = double(y) - double(cy), zd = double(z) - double(cz);
-    Geom<double> g = geometry<double>(xd, yd, zd, double(s.mass), s.spin_d, s.spin_d * s.spin_d,
-                                      s.charge2_d, double(s.chart));
-    double lz = xd * double(py) - yd * double(px);
-    double o3[3];
-    if (g.ok && surface_observable<double>(double(s.mass), s.spin_d, s.charge2_d, double(s.chart),
-                                           double(s.omega), xd, yd, g.r, double(E), lz, o3)) {
-      ob[0] = float(o3[0]);
-      ob[1] = float(o3[1]);
-      ob[2] = float(o3[2]);
-    } else {
-      flags |= kFlagDenominator;
-    }
+    const Anchor a = surface_anchor(s, x, y, z, cx, cy, cz, px, py, E);
+    ob[0] = a.r;
+    ob[1] = a.phi;
+    ob[2] = a.g;
+    if (!a.ok) flags |= kFlagDenominator;
   }
   if (o.status) o.status[idx] = (term & 15u) | ((candidates & 15u) << 4) | ((flags & 255u) << 8);
   if (o.observable) {

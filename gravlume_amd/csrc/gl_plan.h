@@ -66,6 +66,8 @@ struct gl_trace_plan {
   // Escape-direction stencil nodes and the shadow-edge pixel list of the
   // accelerated sky trace and coverage refinement (gl_accel.hip).
   float4* d_nodes = nullptr;
+  uint32_t* d_ready = nullptr;  // per-node epoch flags of the fused stencil launch
+  uint32_t sky_epoch = 0;
   size_t node_capacity = 0;
   uint32_t* d_edges = nullptr;  // [0] count, [1..] pixel indices
   size_t edge_capacity = 0;

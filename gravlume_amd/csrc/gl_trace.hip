@@ -840,6 +840,7 @@ gl_status gl_trace_plan_destroy(gl_trace_plan* p) {
   if (p->d_map) (void)hipFree(p->d_map);
   if (p->d_lut) (void)hipFree(p->d_lut);
   if (p->d_nodes) (void)hipFree(p->d_nodes);
+  if (p->d_ready) (void)hipFree(p->d_ready);
   if (p->d_edges) (void)hipFree(p->d_edges);
   if (p->d_queue) (void)hipFree(p->d_queue);
   if (p->d_frame) (void)hipFree(p->d_frame);

@@ -62,7 +62,8 @@ struct alignas(64) QueueDev {
   unsigned long long refills;
   unsigned long long attempts;  // DP5 attempts by active lanes
   unsigned long long slots;     // lane-attempt slots offered
-  unsigned long long pad[3];
+  unsigned long long aux;       // second cursor (the accelerated sky trace's nodes)
+  unsigned long long pad[2];
 };
 
 // Output planes as the kernels see them (see gl_ray_outputs in gravlume.h).

@@ -197,6 +197,37 @@ class TracePipeline:
                                                          _stream_handle(stream)))
         return n.value
 
+    def surface_footprint(self, scene: int = 0, pixels=None, stream=None) -> dict:
+        """Source-chart Jacobians d(r, r_c dphi)/d(pixel x, y) of surface
+        pixels by +-0.25-pixel central differences of real traces
+        (surface_footprint_v1, footprint.rs:74-151). `pixels`: indices
+        y * width + x (default: every pixel). Returns numpy arrays: resolved,
+        parity (1 positive, 2 negative, 3 degenerate), J (n, 2, 2) with rows
+        (radial, local arc) and columns (x, y), singular values (n, 2)."""
+        n_all = self.scene_rays(scene)
+        if pixels is None:
+            d_pix, n = None, n_all
+        else:
+            d_pix = torch.as_tensor(np.asarray(pixels, dtype=np.int32), device="cuda")
+            n = int(d_pix.numel())
+        jac = torch.zeros(4 * max(n, 1), dtype=torch.float32, device="cuda")
+        meta = torch.zeros(max(n, 1), dtype=torch.int32, device="cuda")
+        _capi.check(self._lib.gl_trace_surface_footprint(
+            self._handle, scene, None if d_pix is None else d_pix.data_ptr(), n, jac.data_ptr(),
+            meta.data_ptr(), _stream_handle(stream)))
+        torch.cuda.synchronize()
+        J = jac.cpu().numpy()[:4 * n].astype(np.float64).reshape(n, 2, 2)
+        m = meta.cpu().numpy()[:n].view(np.uint32)
+        det = J[:, 0, 0] * J[:, 1, 1] - J[:, 0, 1] * J[:, 1, 0]
+        norm2 = np.sum(J * J, axis=(1, 2))
+        major = np.sqrt(0.5 * (norm2 + np.sqrt(np.maximum(norm2 * norm2 - 4 * det * det, 0.0))))
+        with np.errstate(divide="ignore", invalid="ignore"):
+            minor = np.where(major == 0.0, 0.0, np.abs(det) / major)
+        return dict(resolved=(m & 1).astype(bool), parity=((m >> 1) & 3).astype(np.int32), J=J,
+                    singular=np.stack([major, minor], axis=1),
+                    termination=((m >> 8) & 15).astype(np.int32),
+                    flags=((m >> 16) & 255).astype(np.int32))
+
     def set_persistent_config(self, block_size=256, blocks_per_cu=2, refill_interval=8,
                               refill_threshold=16, waves_per_simd=2) -> None:
         c = _capi.PersistentConfig(block_size, blocks_per_cu, refill_interval, refill_threshold,

@@ -255,6 +255,20 @@ gl_status gl_trace_refine_shadow(gl_trace_plan* plan, uint32_t scene, const gl_r
 /* Edge pixels listed by the last gl_trace_refine_shadow (synchronizes the stream). */
 gl_status gl_trace_plan_shadow_edges(gl_trace_plan* plan, uint64_t* count, void* stream);
 
+/* Source-chart footprint of surface pixels (surface_footprint_v1,
+ * crates/gravlume-reference/src/footprint.rs:74-151; GPU capture
+ * surface_footprint_capture.wgsl:1-99): for each pixel (the device list
+ * `pixels`, indices y * width + x, or every pixel when NULL and n equals the
+ * pixel count) the centre and its four +-0.25-pixel neighbours are traced;
+ * when all five end unambiguously on the surface (no flags, candidate mask
+ * exactly the surface bit) with one branch key, jacobian[4i..4i+3] =
+ * (dr/dx, dr/dy, d(r_c dphi)/dx, d(r_c dphi)/dy) in M per pixel by central
+ * differences, else zeros. meta[i]: bit 0 resolved, bits 1-2 parity (1
+ * positive, 2 negative, 3 degenerate), bits 8-11 the centre's termination,
+ * bits 16-23 its flags. Device outputs. */
+gl_status gl_trace_surface_footprint(gl_trace_plan* plan, uint32_t scene, const uint32_t* pixels,
+                                     size_t n, float* jacobian, uint32_t* meta, void* stream);
+
 /* Launch parameters of the persistent kernel (set by the autotuner). */
 typedef struct gl_persistent_config {
   uint32_t block_size;       /* lanes per workgroup: 64, 128, 192 or 256 */

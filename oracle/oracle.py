@@ -438,3 +438,64 @@ def outcomes_to_arrays(outs) -> dict:
         else:  # fixed arrays of double
             res[name] = raw.view(np.float64).reshape(n, -1)
     return res
+
+
+def _wrap(d):
+    return (d + np.pi) % (2 * np.pi) - np.pi
+
+
+FOOTPRINT_OFFSETS = ((0.0, 0.0), (-0.25, 0.0), (0.25, 0.0), (0.0, -0.25), (0.0, 0.25))
+
+
+def footprint_metrics(J):
+    """Determinant, squared norm and singular values of 2x2 Jacobians
+    (footprint.rs:208-238): the minor axis as |det| / major."""
+    J = np.asarray(J, dtype=np.float64)
+    det = J[..., 0, 0] * J[..., 1, 1] - J[..., 0, 1] * J[..., 1, 0]
+    norm2 = np.sum(J * J, axis=(-2, -1))
+    disc = np.sqrt(np.maximum(norm2 * norm2 - 4.0 * det * det, 0.0))
+    major = np.sqrt(0.5 * (norm2 + disc))
+    with np.errstate(divide="ignore", invalid="ignore"):
+        minor = np.where(major == 0.0, 0.0, np.abs(det) / major)
+    return det, norm2, major, minor
+
+
+def surface_footprint(obs, pol, ev, emitter, px, py, subx=0.5, suby=0.5, threads=8):
+    """ObservationTracer::surface_footprint_v1 (crates/gravlume-reference/src/
+    footprint.rs:74-151): the source-chart Jacobian d(r, r_c dphi)/d(pixel x,
+    y) by central differences of real traces at +-0.25 pixel, resolved only
+    when the centre and all four neighbours end unambiguously on the surface
+    with the same branch key. Returns dict(status (0 resolved, 1
+    discontinuity, 2 centre unavailable), J (n, 2, 2), parity (1 positive,
+    2 negative, 3 degenerate), singular (n, 2))."""
+    px = np.atleast_1d(np.asarray(px, dtype=np.float64))
+    py = np.atleast_1d(np.asarray(py, dtype=np.float64))
+    n = px.size
+    sxy = np.stack([sight_planes(obs, px, py, subx + ox, suby + oy) for ox, oy in FOOTPRINT_OFFSETS],
+                   axis=1)
+    outs, st = observation_trace_ordered(obs, pol, ev, sxy.reshape(-1, 2), emitter, threads)
+    A = outcomes_to_arrays(outs)
+    term = A["termination"].reshape(n, 5)
+    cand = A["candidates"].reshape(n, 5)
+    single = (cand != 0) & ((cand & (cand - 1)) == 0)
+    ok = (term == TERM_SURFACE) & (A["surface_status"].reshape(n, 5) == 0) & single & \
+        (st.reshape(n, 5) == 0)
+    key = np.stack([A["polar_side"], A["radial_turnings"].astype(np.int64),
+                    A["equatorial_crossings"].astype(np.int64), A["azimuth_winding"]], axis=-1)
+    key = key.reshape(n, 5, 4)
+    same = np.all(key == key[:, :1, :], axis=(1, 2))
+    r = A["source_radius"].reshape(n, 5)
+    phi = A["source_azimuth"].reshape(n, 5)
+    arc = r[:, :1] * _wrap(phi - phi[:, :1])
+    J = np.zeros((n, 2, 2))
+    J[:, 0, 0] = (r[:, 2] - r[:, 1]) / 0.5
+    J[:, 0, 1] = (r[:, 4] - r[:, 3]) / 0.5
+    J[:, 1, 0] = (arc[:, 2] - arc[:, 1]) / 0.5
+    J[:, 1, 1] = (arc[:, 4] - arc[:, 3]) / 0.5
+    status = np.where(~ok[:, 0], 2, np.where(np.all(ok, axis=1) & same, 0, 1))
+    J[status != 0] = 0.0
+    det, norm2, major, minor = footprint_metrics(J)
+    floor = 64.0 * np.finfo(np.float64).eps * norm2
+    parity = np.where(det > floor, 1, np.where(det < -floor, 2, 3))
+    return dict(status=status, J=J, parity=parity, singular=np.stack([major, minor], axis=1))
+
