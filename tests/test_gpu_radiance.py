@@ -144,3 +144,35 @@ def test_frame_radiance_is_the_transport_of_the_traced_observables(gl, oracle, m
     assert np.all(px[t == 1] == 0.0)
     assert np.array_equal(h["rgba16f"][srf, :3].view(np.uint16),
                           h["radiance"][srf].astype(np.float16).view(np.uint16))
+
+
+def test_fixture_documents_trace_on_the_gpu(gl):
+    """The five surface fixture documents, read by the strict fixture reader
+    (gravlume_amd/fixture.py) and traced as scenes: the sampled pixel's
+    source anchor, frequency ratio and travel time within this build's f32
+    parity bounds, and its radiance within the reference's GPU budgets."""
+    import glob
+    from gravlume_amd import fixture as F
+    root = os.path.join(HERE, "golden", "fixtures")
+    presets = F.Presets(root)
+    docs = [F.parse_toml(open(p, encoding="utf-8").read(), presets)
+            for p in sorted(glob.glob(os.path.join(root, "v[23]", "*.toml")))]
+    assert len(docs) == 5
+    for doc in docs:
+        sc = doc.scene()
+        h = _trace(gl, sc)
+        x, y = doc.sample[0], doc.sample[1]
+        i = y * sc.width + x
+        e = doc.expected
+        assert h["termination"][i] == 7, doc.id
+        r, phi, g = (float(v) for v in h["observable"][i])
+        assert abs(r - e["source_radius_m"]) <= 1e-3, doc.id
+        assert abs(phi - e["source_azimuth_rad"]) <= 1e-4, doc.id
+        assert abs(g / e["frequency_ratio"] - 1) <= 1e-4, doc.id
+        assert abs(h["travel_time"][i] / e["travel_time_m"] - 1) <= 1e-4, doc.id
+        got = h["radiance"][i].astype(np.float64)
+        if doc.schema_version == 3:
+            want = np.array(e["observed_spectral_band_intensities"])
+            assert np.all(np.abs(got / want - 1) <= SPECTRAL_BUDGET), doc.id
+        else:
+            assert np.all(np.abs(got / e["observed_bolometric_intensity"] - 1) <= BOLOMETRIC_BUDGET), doc.id
