@@ -77,6 +77,8 @@ class SceneDesc(C.Structure):
         ("slab_temperature", C.c_double), ("rtol", C.c_double), ("atol", C.c_double),
         ("initial_step", C.c_double), ("minimum_step", C.c_double),
         ("maximum_step", C.c_double), ("maximum_step_per_radius", C.c_double),
+        ("step_safety", C.c_double), ("step_min_factor", C.c_double),
+        ("step_max_factor", C.c_double),
         ("max_accepted_steps", C.c_uint32), ("max_consecutive_rejects", C.c_uint32),
         ("event_tie_tolerance", C.c_double), ("arming_band", C.c_double),
         ("drift_limit", C.c_double), ("reproject_interval", C.c_uint32),

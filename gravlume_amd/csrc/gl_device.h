@@ -364,6 +364,7 @@ __device__ __forceinline__ void hold_in_vgprs(SceneT<float>& sc) {
                "+v"(sc.disk_out));
   asm volatile("" : "+v"(sc.band), "+v"(sc.rtol), "+v"(sc.atol), "+v"(sc.h_min), "+v"(sc.h_max),
                "+v"(sc.h_max_per_r), "+v"(sc.tie_tol), "+v"(sc.drift_limit), "+v"(sc.guard));
+  asm volatile("" : "+v"(sc.safety), "+v"(sc.min_factor), "+v"(sc.max_factor));
 }
 
 // Null re-projection on a schedule every lane of a wavefront shares: the

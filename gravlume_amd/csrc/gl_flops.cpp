@@ -134,6 +134,9 @@ uint64_t count_attempt() {
   sc.h_min = 1e-6;
   sc.h_max = 64.0;
   sc.h_max_per_r = 1.0;
+  sc.safety = 0.9;
+  sc.min_factor = 0.2;
+  sc.max_factor = 5.0;
   sc.tie_tol = 1.0 / 131072.0;
   sc.drift_limit = 0.05;
   sc.guard = std::ldexp(1.0, -40);

@@ -192,7 +192,8 @@ int prepare_scene(const gl_scene_desc& d, SceneDev* out, SceneHost* host, std::s
                          d.escape_radius, d.horizon_offset, d.rtol, d.atol, d.initial_step,
                          d.minimum_step, d.maximum_step, d.maximum_step_per_radius,
                          d.event_tie_tolerance, d.arming_band, d.drift_limit,
-                         d.subpixel[0], d.subpixel[1]};
+                         d.subpixel[0], d.subpixel[1], d.step_safety, d.step_min_factor,
+                         d.step_max_factor};
   for (double v : vals)
     if (!std::isfinite(v)) return set_err(err, "scene value is not finite"), GL_ERR_INVALID_ARGUMENT;
   for (int i = 0; i < 4; ++i)
@@ -213,6 +214,15 @@ int prepare_scene(const gl_scene_desc& d, SceneDev* out, SceneHost* host, std::s
   if (!(d.rtol > 0.0 && d.atol > 0.0 && d.minimum_step > 0.0 && d.maximum_step >= d.minimum_step &&
         d.initial_step >= d.minimum_step && d.initial_step <= d.maximum_step))
     return set_err(err, "invalid step policy"), GL_ERR_INVALID_ARGUMENT;
+  {
+    // controller factors (policy.rs:38-66): 0 < safety <= 1, 0 < min <= 1 <= max
+    const double sf = d.step_safety == 0.0 ? 0.9 : d.step_safety;
+    const double mn = d.step_min_factor == 0.0 ? 0.2 : d.step_min_factor;
+    const double mx = d.step_max_factor == 0.0 ? 5.0 : d.step_max_factor;
+    if (!(sf > 0.0 && sf <= 1.0) || !(mn > 0.0 && mn <= 1.0) || !(mx >= 1.0 && mx <= 1.0e3))
+      return set_err(err, "step controller factors must satisfy 0 < safety <= 1, 0 < min <= 1 <= max"),
+             GL_ERR_INVALID_ARGUMENT;
+  }
   if (d.max_accepted_steps == 0 || d.max_consecutive_rejects == 0)
     return set_err(err, "step limits must be positive"), GL_ERR_INVALID_ARGUMENT;
   if (!(d.horizon_offset >= 0.0 && d.horizon_offset < 1.0))
@@ -339,6 +349,9 @@ int prepare_scene(const gl_scene_desc& d, SceneDev* out, SceneHost* host, std::s
   s.h_min = (float)d.minimum_step;
   s.h_max = (float)d.maximum_step;
   s.h_max_per_r = (float)d.maximum_step_per_radius;
+  s.safety = d.step_safety == 0.0 ? 0.9f : (float)d.step_safety;
+  s.min_factor = d.step_min_factor == 0.0 ? 0.2f : (float)d.step_min_factor;
+  s.max_factor = d.step_max_factor == 0.0 ? 5.0f : (float)d.step_max_factor;
   s.tie_tol = (float)d.event_tie_tolerance;
   s.drift_limit = (float)d.drift_limit;
   s.guard = (float)std::ldexp(1.0, -40);  // (r^4 + a^2 z^2) / M^4 = 2^-40 (policy.rs:3)

@@ -50,7 +50,8 @@ struct alignas(16) SceneDev {
   double slab_emission_d;  // integrated slab emission (0 in vacuum)
   double slab_temperature_d;  // its blackbody temperature [K], 0 neutral
   const float* lut;        // 4097 x 4 log2 band fractions (blackbody scenes)
-  uint32_t pad_[4];
+  float safety, min_factor, max_factor;  // step controller factors
+  uint32_t pad_;
 };
 
 static_assert(sizeof(SceneDev) == 320, "SceneDev must stay 320 bytes");

@@ -116,6 +116,7 @@ struct SceneT {
   T mass, spin, spin2, charge2, chart;
   T r_capture, r_horizon, r_escape, disk_in, disk_out;
   T band, rtol, atol, h_min, h_max, h_max_per_r, tie_tol, drift_limit, guard;
+  T safety, min_factor, max_factor;  // step controller (policy.rs:160-173)
   uint32_t max_accepted, max_rejects, options, reproject_interval;
 };
 
@@ -143,6 +144,9 @@ GL_HD SceneT<float> scene_f32(const SceneDev& s) {
   c.tie_tol = s.tie_tol;
   c.drift_limit = s.drift_limit;
   c.guard = s.guard;
+  c.safety = s.safety;
+  c.min_factor = s.min_factor;
+  c.max_factor = s.max_factor;
   c.max_accepted = s.max_accepted;
   c.max_rejects = s.max_rejects;
   c.options = s.options;
@@ -171,6 +175,9 @@ GL_HD SceneT<double> scene_f64(const SceneDev& s) {
   c.tie_tol = s.tie_tol;
   c.drift_limit = s.drift_limit;
   c.guard = s.guard;
+  c.safety = s.safety;
+  c.min_factor = s.min_factor;
+  c.max_factor = s.max_factor;
   c.max_accepted = s.max_accepted;
   c.max_rejects = s.max_rejects;
   c.options = s.options;
@@ -762,7 +769,7 @@ GL_HD void step(const SceneT<T>& sc, RayT<T>& R, T omega, bool reproject = false
       R.term = kTermFailure;
       return;
     }
-    T f = gl_clamp<T>(T(0.9) * pow_m02<T>(en), T(0.2), T(1));
+    T f = gl_clamp<T>(sc.safety * pow_m02<T>(en), sc.min_factor, T(1));
     R.h = T(R.dir) * gl_clamp<T>(hmag * f, sc.h_min, step_cap<T>(sc, R.r));
     return;
   }
@@ -899,12 +906,13 @@ GL_HD void step(const SceneT<T>& sc, RayT<T>& R, T omega, bool reproject = false
       return;
     }
   }
-  // Step control of policy.rs:160-173 (safety 0.9, factor in [0.2, 5]) with
+  // Step control of policy.rs:160-173 (safety 0.9, factor in [0.2, 5] by
+  // default: the scene's safety, min_factor, max_factor) with
   // no growth on the step right after a rejection (Hairer, Norsett & Wanner,
   // Solving ODEs I, II.4), which breaks the accept/reject alternation of
   // rays falling inward.
-  T f = en == T(0) ? T(5) : T(0.9) * pow_m02<T>(en);
-  f = gl_clamp<T>(f, T(0.2), after_reject ? T(1) : T(5));
+  T f = en == T(0) ? sc.max_factor : sc.safety * pow_m02<T>(en);
+  f = gl_clamp<T>(f, sc.min_factor, after_reject ? T(1) : sc.max_factor);
   R.h = dir * gl_clamp<T>(hmag * f, sc.h_min, step_cap<T>(sc, R.r));
 }
 

@@ -165,6 +165,9 @@ class Scene:
     minimum_step: float = 2.0 ** -30
     maximum_step: float = 256.0
     maximum_step_per_radius: float = 0.25
+    step_safety: float = 0.9       # controller safety factor (policy.rs:160-173)
+    step_min_factor: float = 0.2
+    step_max_factor: float = 5.0
     max_accepted_steps: int = 100_000
     max_consecutive_rejects: int = 64
     event_tie_tolerance: float = 2.0 ** -17
@@ -244,6 +247,9 @@ class Scene:
         d.minimum_step = self.minimum_step
         d.maximum_step = self.maximum_step
         d.maximum_step_per_radius = self.maximum_step_per_radius
+        d.step_safety = self.step_safety
+        d.step_min_factor = self.step_min_factor
+        d.step_max_factor = self.step_max_factor
         d.max_accepted_steps = self.max_accepted_steps
         d.max_consecutive_rejects = self.max_consecutive_rejects
         d.event_tie_tolerance = self.event_tie_tolerance

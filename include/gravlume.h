@@ -130,6 +130,10 @@ typedef struct gl_scene_desc {
   double rtol, atol;
   double initial_step, minimum_step, maximum_step;
   double maximum_step_per_radius; /* |h| <= this * r (0 disables) */
+  /* step controller h <- h clamp(safety E^-1/5, min_factor, max_factor)
+   * (ReferencePolicy, policy.rs:5-66, 160-173); 0 selects the reference's
+   * 0.9, 0.2 and 5 */
+  double step_safety, step_min_factor, step_max_factor;
   uint32_t max_accepted_steps, max_consecutive_rejects;
   double event_tie_tolerance; /* M */
   double arming_band;         /* M */

@@ -144,3 +144,19 @@ def test_scene_validate_needs_no_device_and_matches_the_oracle_frame_check(oracl
         oracle.observation(1.0, bad.spin, bad.charge, float(bad.chart), [0.0, *bad.observer_xyz()],
                            [0, 0, 0, 0], [0, 0, 1], 1.0, bad.width, bad.height,
                            math.radians(bad.fov_deg))
+
+
+def test_step_controller_factors_are_validated():
+    """The reference policy's controller factors (policy.rs:38-66): 0 < safety
+    <= 1, 0 < min <= 1 <= max; zero selects the reference defaults."""
+    from gravlume_amd import _capi, scene as S
+    lib = _capi.load()
+    good = S.small_cpu_scene(escape_map=False)
+    d = good.to_desc()
+    d.step_safety = d.step_min_factor = d.step_max_factor = 0.0
+    assert lib.gl_scene_validate(C.byref(d)) == 0
+    for field, value in (("step_safety", 1.5), ("step_min_factor", -0.1), ("step_max_factor", 0.5),
+                         ("step_safety", float("nan"))):
+        d = good.to_desc()
+        setattr(d, field, value)
+        assert lib.gl_scene_validate(C.byref(d)) == _capi.GL_ERR_INVALID_ARGUMENT, field
