@@ -5,8 +5,13 @@ ABI of include/gravlume.h; this package is the host-side mirror of the
 reference's trace seams. Importing it loads libgravlume_hip.so and fails if
 the library has not been built: there is no CPU fallback.
 """
-from . import _capi
-from ._capi import GravlumeError
+# PyTorch first: its HIP runtime is the process's runtime, and the library
+# binds to the libamdhip64 already loaded. Loaded the other way round, the
+# library's runtime can find no device on a host where PyTorch does.
+import torch  # noqa: F401,E402
+
+from . import _capi  # noqa: E402
+from ._capi import GravlumeError  # noqa: E402
 
 _capi.load()
 
