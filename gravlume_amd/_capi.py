@@ -153,6 +153,7 @@ SIGNATURES = {
     "gl_vjp_forward": (C.c_int, [_P, C.c_uint32, C.c_double, C.c_uint32,
                                  C.POINTER(RayOutputsC), _P, C.POINTER(_P)]),
     "gl_vjp_backward": (C.c_int, [_P, _P, C.POINTER(C.c_double), _P, _P]),
+    "gl_vjp_backward_observables": (C.c_int, [_P, _P, _P, _P, C.POINTER(C.c_double), _P, _P]),
     "gl_vjp_destroy": (C.c_int, [_P]),
     "gl_last_error": (C.c_char_p, []),
     "gl_version": (C.c_char_p, []),

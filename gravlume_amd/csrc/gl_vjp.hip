@@ -102,11 +102,13 @@ __global__ void __launch_bounds__(64) k_vjp_forward(const SceneDev* __restrict__
   write_ray(R, out, idx, s, map);
 }
 
-// Adjoint of the terminal (surface-crossing) step, seeded by gbar = dL/dg.
-// Returns false when the observable is not differentiable there.
+// Adjoint of the terminal (surface-crossing) step, seeded by dL/d(r, phi_s,
+// g) of the surface observable. Returns false when the observable is not
+// differentiable there.
 __device__ __forceinline__ bool terminal_adjoint(const SceneT<float>& sc, const SceneDev& s,
-                                                 const float y[6], float E, float h, float gbar,
-                                                 float lam[6], RhsBar<float>& acc) {
+                                                 const float y[6], float E, float h, float wr,
+                                                 float wphi, float wg, float lam[6],
+                                                 RhsBar<float>& acc) {
   float K[7][6], Kt[7][7];
   const float th = terminal_stages<float>(sc, y, E, h, K, Kt);
   float yT[6], dtl;
@@ -121,17 +123,22 @@ __device__ __forceinline__ bool terminal_adjoint(const SceneT<float>& sc, const 
                                  Dual5(double(yT[0])), Dual5(double(yT[1])), Dual5::var(gT.r, 0),
                                  Dual5::var(E, 1), Dual5::var(lz, 2), o))
     return false;
-  const float gr = o[2].d[0], gE = o[2].d[1], glz = o[2].d[2];
+  // g depends on (r, E, L_z, a, Q^2); phi_s = atan2(y, x) - atan2(s a, r) on
+  // (x, y) directly and on (r, a) through the duals; r on the position and a^2
+  const float glz = o[2].d[2];
+  const float cr = wr + wphi * float(o[1].d[0]) + wg * float(o[2].d[0]);  // dL/dr in total
   const float rr = gT.r;
   const float rho2 = yT[0] * yT[0] + yT[1] * yT[1] + yT[2] * yT[2];
   const float dr_da2 = -(rr * rr - yT[2] * yT[2]) /
                        (4.0f * rr * rr * rr - 2.0f * rr * (rho2 - sc.spin2));
-  const float lT[6] = {gbar * (gr * gT.grx + glz * yT[4]), gbar * (gr * gT.gry - glz * yT[3]),
-                       gbar * gr * gT.grz, -gbar * glz * yT[1], gbar * glz * yT[0], 0.0f};
-  acc.E += gbar * gE;
-  acc.spin += gbar * o[2].d[3];
-  acc.spin2 += gbar * gr * dr_da2;
-  acc.charge2 += gbar * o[2].d[4];
+  const float rxy2 = yT[0] * yT[0] + yT[1] * yT[1];
+  const float px = -wphi * yT[1] / rxy2, py = wphi * yT[0] / rxy2;  // d(atan2(y, x))
+  const float lT[6] = {cr * gT.grx + wg * glz * yT[4] + px, cr * gT.gry - wg * glz * yT[3] + py,
+                       cr * gT.grz, -wg * glz * yT[1], wg * glz * yT[0], 0.0f};
+  acc.E += wg * float(o[2].d[1]);
+  acc.spin += wg * float(o[2].d[3]) + wphi * float(o[1].d[3]);
+  acc.spin2 += cr * dr_da2;
+  acc.charge2 += wg * float(o[2].d[4]) + wphi * float(o[1].d[4]);
   return terminal_reverse<float>(sc, y, E, h, K, th, lT, lam, acc);
 }
 
@@ -196,6 +203,8 @@ __device__ __noinline__ Grad5 initial_contract(const SceneDev& s, const VjpTange
 
 __global__ void __launch_bounds__(64) k_vjp_backward(const SceneDev* __restrict__ scenes,
                                                      uint32_t scene, VjpBufs vb,
+                                                     const float* __restrict__ grad_r,
+                                                     const float* __restrict__ grad_phi,
                                                      const float* __restrict__ grad_g,
                                                      float* __restrict__ grad_dir,
                                                      const VjpTangents* __restrict__ tg,
@@ -207,16 +216,18 @@ __global__ void __launch_bounds__(64) k_vjp_backward(const SceneDev* __restrict_
   const size_t rays = vb.rays;
   const uint32_t idx = valid ? py * s.width + px : 0u;
   const uint32_t N = valid ? vb.n[idx] : 0u;
-  const float gbar = valid ? grad_g[idx] : 0.0f;
+  const float wr = valid && grad_r ? grad_r[idx] : 0.0f;
+  const float wphi = valid && grad_phi ? grad_phi[idx] : 0.0f;
+  const float wg = valid && grad_g ? grad_g[idx] : 0.0f;
   float gp[3] = {0.0f, 0.0f, 0.0f};
   float gd[2] = {0.0f, 0.0f};
-  if (N > 0u && gbar != 0.0f) {
+  if (N > 0u && (wr != 0.0f || wphi != 0.0f || wg != 0.0f)) {
     const float E = vb.E[idx];
     RhsBar<float> acc = {};
     float y[6], lam[6];
 #pragma unroll
     for (int c = 0; c < 6; ++c) y[c] = vb.term[size_t(c) * rays + idx];
-    if (terminal_adjoint(sc, s, y, E, vb.h[size_t(N - 1) * rays + idx], gbar, lam, acc)) {
+    if (terminal_adjoint(sc, s, y, E, vb.h[size_t(N - 1) * rays + idx], wr, wphi, wg, lam, acc)) {
       if (N >= 2u) {
         for (int m = int((N - 2u) / vb.K); m >= 0; --m) {
           const uint32_t n0 = uint32_t(m) * vb.K;
@@ -420,13 +431,24 @@ gl_status gl_vjp_forward(gl_trace_plan* p, uint32_t scene, double inclination,
 
 gl_status gl_vjp_backward(gl_vjp_state* st, const float* grad_g, double* grad_params,
                           float* grad_direction, void* stream) {
-  if (!st || !grad_g || !grad_params)
-    return fail(GL_ERR_INVALID_ARGUMENT, "state, grad_g and grad_params are required");
+  if (!grad_g) return fail(GL_ERR_INVALID_ARGUMENT, "grad_g is required");
+  return gl_vjp_backward_observables(st, nullptr, nullptr, grad_g, grad_params, grad_direction,
+                                     stream);
+}
+
+gl_status gl_vjp_backward_observables(gl_vjp_state* st, const float* grad_r,
+                                      const float* grad_phi, const float* grad_g,
+                                      double* grad_params, float* grad_direction, void* stream) {
+  if (!st || !grad_params)
+    return fail(GL_ERR_INVALID_ARGUMENT, "state and grad_params are required");
+  if (!grad_r && !grad_phi && !grad_g)
+    return fail(GL_ERR_INVALID_ARGUMENT, "at least one observable seed is required");
   hipStream_t hs_ = (hipStream_t)stream;
   GL_HIP(hipMemsetAsync(st->d_grad, 0, 3 * sizeof(double), hs_));
   plan_mark(st->plan, 0, hs_);
   hipLaunchKernelGGL(k_vjp_backward, dim3(st->tiles), dim3(64), 0, hs_, st->plan->d_scenes,
-                     st->scene, st->vb, grad_g, grad_direction, st->d_tangents, st->d_grad);
+                     st->scene, st->vb, grad_r, grad_phi, grad_g, grad_direction, st->d_tangents,
+                     st->d_grad);
   plan_mark(st->plan, 1, hs_);
   GL_HIP(hipGetLastError());
   GL_HIP(hipMemcpyAsync(st->h_grad, st->d_grad, 3 * sizeof(double), hipMemcpyDeviceToHost, hs_));

@@ -423,6 +423,52 @@ class VjpTrace:
             return params
         return params, gdir.view(2, -1)
 
+    def backward_observables(self, grad_r=None, grad_phi=None, grad_g=None,
+                             direction: bool = False, stream=None):
+        """dL/d(a, Q, inclination) (and per-pixel dL/d(s_x, s_y)) for
+        L = sum_pixels (grad_r r + grad_phi phi_s + grad_g g) over surface hits."""
+        dev = self.target.status.device
+        seeds = []
+        for v in (grad_r, grad_phi, grad_g):
+            if v is None:
+                seeds.append(None)
+                continue
+            t = torch.as_tensor(v).detach().to(device=dev, dtype=torch.float32).contiguous()
+            if t.numel() != self.target.rays:
+                raise ValueError("a seed needs one value per pixel")
+            seeds.append(t)
+        gp = (C.c_double * 3)()
+        gdir = torch.zeros(2 * self.target.rays, dtype=torch.float32, device=dev) if direction else None
+        ptr = [C.c_void_p(t.data_ptr()) if t is not None else None for t in seeds]
+        _capi.check(self.pipeline._lib.gl_vjp_backward_observables(
+            self._handle, ptr[0], ptr[1], ptr[2], gp,
+            C.c_void_p(gdir.data_ptr()) if gdir is not None else None, _stream_handle(stream)))
+        params = np.array([gp[0], gp[1], gp[2]], dtype=np.float64)
+        if gdir is None:
+            return params
+        return params, gdir.view(2, -1)
+
+    def lensing_jacobian(self) -> np.ndarray:
+        """Per-pixel source-chart Jacobian d(r, r phi_s)/d(pixel x, y), shape
+        (n, 2, 2), rows (radial, local arc), columns (x, y), from two adjoint
+        passes (seed r, then phi_s) and the pixel-to-sight-plane scale; zero
+        off the surface. The analytic counterpart of the +-0.25-pixel central
+        differences of surface_footprint (footprint.rs:74-151)."""
+        sc = self.pipeline.scenes[self.scene]
+        n = self.target.rays
+        ones = torch.ones(n, dtype=torch.float32, device=self.target.status.device)
+        _, dr = self.backward_observables(grad_r=ones, direction=True)
+        _, dphi = self.backward_observables(grad_phi=ones, direction=True)
+        th = math.tan(0.5 * math.radians(sc.fov_deg))
+        dsx = 2.0 * (sc.width / sc.height) * th / sc.width   # d s_x / d pixel x
+        dsy = -2.0 * th / sc.height                          # d s_y / d pixel y
+        r = self.target.observable[:n].double().cpu().numpy()
+        dr, dphi = dr.double().cpu().numpy(), dphi.double().cpu().numpy()
+        J = np.zeros((n, 2, 2))
+        J[:, 0, 0], J[:, 0, 1] = dr[0] * dsx, dr[1] * dsy
+        J[:, 1, 0], J[:, 1, 1] = r * dphi[0] * dsx, r * dphi[1] * dsy
+        return J
+
     def close(self) -> None:
         if self._handle:
             self.pipeline._lib.gl_vjp_destroy(self._handle)

@@ -357,6 +357,13 @@ gl_status gl_vjp_forward(gl_trace_plan* plan, uint32_t scene, double inclination
  * grad_direction: device float[2 * n_rays] (d/d sight-plane x, y), may be NULL. */
 gl_status gl_vjp_backward(gl_vjp_state* state, const float* grad_g, double* grad_params,
                           float* grad_direction, void* stream);
+/* The same for a seed on any of the surface observables: grad_r, grad_phi,
+ * grad_g are device float[n_rays] of dL/dr, dL/dphi_s, dL/dg (NULL = 0, at
+ * least one given). Seeding r and then phi_s gives every surface pixel's
+ * lensing Jacobian d(r, phi_s)/d(sight-plane x, y) in two passes. */
+gl_status gl_vjp_backward_observables(gl_vjp_state* state, const float* grad_r,
+                                      const float* grad_phi, const float* grad_g,
+                                      double* grad_params, float* grad_direction, void* stream);
 gl_status gl_vjp_destroy(gl_vjp_state* state);
 
 /* Per-thread text of the last error. */

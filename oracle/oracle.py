@@ -460,7 +460,8 @@ def footprint_metrics(J):
     return det, norm2, major, minor
 
 
-def surface_footprint(obs, pol, ev, emitter, px, py, subx=0.5, suby=0.5, threads=8):
+def surface_footprint(obs, pol, ev, emitter, px, py, subx=0.5, suby=0.5, threads=8,
+                      offset=0.25):
     """ObservationTracer::surface_footprint_v1 (crates/gravlume-reference/src/
     footprint.rs:74-151): the source-chart Jacobian d(r, r_c dphi)/d(pixel x,
     y) by central differences of real traces at +-0.25 pixel, resolved only
@@ -471,8 +472,9 @@ def surface_footprint(obs, pol, ev, emitter, px, py, subx=0.5, suby=0.5, threads
     px = np.atleast_1d(np.asarray(px, dtype=np.float64))
     py = np.atleast_1d(np.asarray(py, dtype=np.float64))
     n = px.size
-    sxy = np.stack([sight_planes(obs, px, py, subx + ox, suby + oy) for ox, oy in FOOTPRINT_OFFSETS],
-                   axis=1)
+    k = offset / 0.25
+    sxy = np.stack([sight_planes(obs, px, py, subx + k * ox, suby + k * oy)
+                    for ox, oy in FOOTPRINT_OFFSETS], axis=1)
     outs, st = observation_trace_ordered(obs, pol, ev, sxy.reshape(-1, 2), emitter, threads)
     A = outcomes_to_arrays(outs)
     term = A["termination"].reshape(n, 5)
@@ -488,10 +490,11 @@ def surface_footprint(obs, pol, ev, emitter, px, py, subx=0.5, suby=0.5, threads
     phi = A["source_azimuth"].reshape(n, 5)
     arc = r[:, :1] * _wrap(phi - phi[:, :1])
     J = np.zeros((n, 2, 2))
-    J[:, 0, 0] = (r[:, 2] - r[:, 1]) / 0.5
-    J[:, 0, 1] = (r[:, 4] - r[:, 3]) / 0.5
-    J[:, 1, 0] = (arc[:, 2] - arc[:, 1]) / 0.5
-    J[:, 1, 1] = (arc[:, 4] - arc[:, 3]) / 0.5
+    span = 2.0 * offset
+    J[:, 0, 0] = (r[:, 2] - r[:, 1]) / span
+    J[:, 0, 1] = (r[:, 4] - r[:, 3]) / span
+    J[:, 1, 0] = (arc[:, 2] - arc[:, 1]) / span
+    J[:, 1, 1] = (arc[:, 4] - arc[:, 3]) / span
     status = np.where(~ok[:, 0], 2, np.where(np.all(ok, axis=1) & same, 0, 1))
     J[status != 0] = 0.0
     det, norm2, major, minor = footprint_metrics(J)
