@@ -154,6 +154,7 @@ SIGNATURES = {
                                  C.POINTER(RayOutputsC), _P, C.POINTER(_P)]),
     "gl_vjp_backward": (C.c_int, [_P, _P, C.POINTER(C.c_double), _P, _P]),
     "gl_vjp_backward_observables": (C.c_int, [_P, _P, _P, _P, C.POINTER(C.c_double), _P, _P]),
+    "gl_vjp_surface_footprint": (C.c_int, [_P, C.POINTER(RayOutputsC), _P, _P, _P]),
     "gl_vjp_destroy": (C.c_int, [_P]),
     "gl_last_error": (C.c_char_p, []),
     "gl_version": (C.c_char_p, []),

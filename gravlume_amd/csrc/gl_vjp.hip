@@ -269,6 +269,53 @@ __global__ void __launch_bounds__(64) k_vjp_backward(const SceneDev* __restrict_
   }
 }
 
+// A seed of one on every ray (the adjoint footprint's two passes).
+__global__ void __launch_bounds__(256) k_fill_ones(float* __restrict__ v, uint32_t n) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i < n) v[i] = 1.0f;
+}
+
+// The source-chart footprint from the two adjoint passes: with
+// dr = d r / d(s_x, s_y) and dphi = d phi_s / d(s_x, s_y) (planar, 2 x n)
+// and the pixel-to-sight-plane scales dsx, dsy,
+//   J = (dr/dx, dr/dy, r dphi/dx, r dphi/dy)   [M per pixel],
+// the rows (radial, local arc) of surface_footprint_v1 (footprint.rs:74-151)
+// without its four neighbour rays. A pixel resolves when the forward ray
+// ended unambiguously on the surface (no flags, candidate mask exactly the
+// surface bit) within the adjoint's step capacity; parity and meta as in
+// k_surface_footprint.
+__global__ void __launch_bounds__(256) k_adjoint_footprint(
+    VjpBufs vb, const uint32_t* __restrict__ status, const float* __restrict__ observable,
+    const float* __restrict__ dr, const float* __restrict__ dphi, float dsx, float dsy,
+    float* __restrict__ jac, uint32_t* __restrict__ meta) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  const uint32_t n = vb.rays;
+  if (i >= n) return;
+  const uint32_t st = status[i];
+  const uint32_t term = st & 15u, cand = (st >> 4) & 15u, flags = (st >> 8) & 255u;
+  float J[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  uint32_t resolved = 0u, parity = 0u;
+  if (vb.n[i] > 0u && term == kTermSurface && flags == 0u && cand == GL_EVENT_SURFACE) {
+    const float r = observable[i];
+    J[0] = dr[i] * dsx;
+    J[1] = dr[n + i] * dsy;
+    J[2] = r * dphi[i] * dsx;
+    J[3] = r * dphi[n + i] * dsy;
+    const float det = J[0] * J[3] - J[1] * J[2];
+    const float norm2 = J[0] * J[0] + J[1] * J[1] + J[2] * J[2] + J[3] * J[3];
+    const float floor_ = 64.0f * 1.1920929e-7f * norm2;
+    parity = det > floor_ ? 1u : (det < -floor_ ? 2u : 3u);
+    resolved = (norm2 == norm2 && norm2 <= 3.0e38f) ? 1u : 0u;
+    if (!resolved) {
+      J[0] = J[1] = J[2] = J[3] = 0.0f;
+      parity = 0u;
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 4; ++c) jac[4 * size_t(i) + c] = J[c];
+  meta[i] = resolved | (parity << 1) | (term << 8) | (flags << 16);
+}
+
 }  // namespace gl
 
 // ===========================================================================
@@ -284,14 +331,18 @@ struct gl_vjp_state {
   VjpTangents* d_tangents = nullptr;
   double* d_grad = nullptr;
   double* h_grad = nullptr;  // pinned
+  // the adjoint footprint's seed and direction gradients (allocated on use)
+  float* d_seed = nullptr;
+  float* d_dr = nullptr;
+  float* d_dphi = nullptr;
 };
 
 namespace {
 
 void release(gl_vjp_state* st) {
   if (!st) return;
-  void* dev[] = {st->vb.h, st->vb.ckpt, st->vb.term, st->vb.E, st->vb.n, st->vb.seg,
-                 st->d_tangents, st->d_grad};
+  void* dev[] = {st->vb.h,       st->vb.ckpt,  st->vb.term, st->vb.E, st->vb.n, st->vb.seg,
+                 st->d_tangents, st->d_grad,   st->d_seed,  st->d_dr, st->d_dphi};
   for (void* p : dev)
     if (p) (void)hipFree(p);
   if (st->h_grad) (void)hipHostFree(st->h_grad);
@@ -454,6 +505,49 @@ gl_status gl_vjp_backward_observables(gl_vjp_state* st, const float* grad_r,
   GL_HIP(hipMemcpyAsync(st->h_grad, st->d_grad, 3 * sizeof(double), hipMemcpyDeviceToHost, hs_));
   GL_HIP(hipStreamSynchronize(hs_));
   for (int k = 0; k < 3; ++k) grad_params[k] = st->h_grad[k];
+  return GL_OK;
+}
+
+gl_status gl_vjp_surface_footprint(gl_vjp_state* st, const gl_ray_outputs* out, float* jacobian,
+                                   uint32_t* meta, void* stream) {
+  if (!st || !out || !jacobian || !meta)
+    return fail(GL_ERR_INVALID_ARGUMENT, "state, outputs, jacobian and meta are required");
+  if (!out->status || !out->observable)
+    return fail(GL_ERR_INVALID_ARGUMENT, "the forward's status and observable planes are required");
+  const uint32_t n = st->vb.rays;
+  if (out->count < n) return fail(GL_ERR_INVALID_ARGUMENT, "output planes hold fewer rays than the scene");
+  hipStream_t hs_ = (hipStream_t)stream;
+  if (!st->d_seed) {
+    hipError_t e = hipMalloc((void**)&st->d_seed, size_t(n) * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc((void**)&st->d_dr, 2 * size_t(n) * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc((void**)&st->d_dphi, 2 * size_t(n) * sizeof(float));
+    if (e != hipSuccess) {
+      (void)hipFree(st->d_seed);
+      (void)hipFree(st->d_dr);
+      (void)hipFree(st->d_dphi);
+      st->d_seed = st->d_dr = st->d_dphi = nullptr;
+      return fail(e == hipErrorOutOfMemory ? GL_ERR_CAPACITY : GL_ERR_DEVICE,
+                  std::string("adjoint footprint buffers: ") + hipGetErrorString(e));
+    }
+  }
+  const dim3 fill((n + 255u) / 256u);
+  hipLaunchKernelGGL(k_fill_ones, fill, dim3(256), 0, hs_, st->d_seed, n);
+  GL_HIP(hipGetLastError());
+  // seed r, then phi_s; the parameter gradients they accumulate are unused
+  hipLaunchKernelGGL(k_vjp_backward, dim3(st->tiles), dim3(64), 0, hs_, st->plan->d_scenes,
+                     st->scene, st->vb, st->d_seed, (const float*)nullptr, (const float*)nullptr,
+                     st->d_dr, st->d_tangents, st->d_grad);
+  GL_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_vjp_backward, dim3(st->tiles), dim3(64), 0, hs_, st->plan->d_scenes,
+                     st->scene, st->vb, (const float*)nullptr, st->d_seed, (const float*)nullptr,
+                     st->d_dphi, st->d_tangents, st->d_grad);
+  GL_HIP(hipGetLastError());
+  const SceneDev& sd = st->plan->scenes[st->scene];
+  const float dsx = 2.0f * sd.aspect * sd.tan_half_fov / float(sd.width);  // d s_x / d pixel x
+  const float dsy = -2.0f * sd.tan_half_fov / float(sd.height);            // d s_y / d pixel y
+  hipLaunchKernelGGL(k_adjoint_footprint, fill, dim3(256), 0, hs_, st->vb, out->status,
+                     out->observable, st->d_dr, st->d_dphi, dsx, dsy, jacobian, meta);
+  GL_HIP(hipGetLastError());
   return GL_OK;
 }
 

@@ -97,6 +97,24 @@ def _stream_handle(stream) -> C.c_void_p:
     return C.c_void_p(stream.cuda_stream)
 
 
+def _footprint_result(jac: torch.Tensor, meta: torch.Tensor, n: int) -> dict:
+    """Host view of a footprint launch: resolved, parity (1 positive,
+    2 negative, 3 degenerate), J (n, 2, 2) with rows (radial, local arc) and
+    columns (x, y), singular values (n, 2) (major; minor = |det| / major),
+    the centre's termination and flags."""
+    J = jac.cpu().numpy()[:4 * n].astype(np.float64).reshape(n, 2, 2)
+    m = meta.cpu().numpy()[:n].view(np.uint32)
+    det = J[:, 0, 0] * J[:, 1, 1] - J[:, 0, 1] * J[:, 1, 0]
+    norm2 = np.sum(J * J, axis=(1, 2))
+    major = np.sqrt(0.5 * (norm2 + np.sqrt(np.maximum(norm2 * norm2 - 4 * det * det, 0.0))))
+    with np.errstate(divide="ignore", invalid="ignore"):
+        minor = np.where(major == 0.0, 0.0, np.abs(det) / major)
+    return dict(resolved=(m & 1).astype(bool), parity=((m >> 1) & 3).astype(np.int32), J=J,
+                singular=np.stack([major, minor], axis=1),
+                termination=((m >> 8) & 15).astype(np.int32),
+                flags=((m >> 16) & 255).astype(np.int32))
+
+
 class TracePipeline:
     """Validated, packed scenes resident on the GPU (TracePipeline::new)."""
 
@@ -216,17 +234,7 @@ class TracePipeline:
             self._handle, scene, None if d_pix is None else d_pix.data_ptr(), n, jac.data_ptr(),
             meta.data_ptr(), _stream_handle(stream)))
         torch.cuda.synchronize()
-        J = jac.cpu().numpy()[:4 * n].astype(np.float64).reshape(n, 2, 2)
-        m = meta.cpu().numpy()[:n].view(np.uint32)
-        det = J[:, 0, 0] * J[:, 1, 1] - J[:, 0, 1] * J[:, 1, 0]
-        norm2 = np.sum(J * J, axis=(1, 2))
-        major = np.sqrt(0.5 * (norm2 + np.sqrt(np.maximum(norm2 * norm2 - 4 * det * det, 0.0))))
-        with np.errstate(divide="ignore", invalid="ignore"):
-            minor = np.where(major == 0.0, 0.0, np.abs(det) / major)
-        return dict(resolved=(m & 1).astype(bool), parity=((m >> 1) & 3).astype(np.int32), J=J,
-                    singular=np.stack([major, minor], axis=1),
-                    termination=((m >> 8) & 15).astype(np.int32),
-                    flags=((m >> 16) & 255).astype(np.int32))
+        return _footprint_result(jac, meta, n)
 
     def set_persistent_config(self, block_size=256, blocks_per_cu=2, refill_interval=8,
                               refill_threshold=16, waves_per_simd=2) -> None:
@@ -448,26 +456,26 @@ class VjpTrace:
             return params
         return params, gdir.view(2, -1)
 
-    def lensing_jacobian(self) -> np.ndarray:
-        """Per-pixel source-chart Jacobian d(r, r phi_s)/d(pixel x, y), shape
-        (n, 2, 2), rows (radial, local arc), columns (x, y), from two adjoint
-        passes (seed r, then phi_s) and the pixel-to-sight-plane scale; zero
-        off the surface. The analytic counterpart of the +-0.25-pixel central
-        differences of surface_footprint (footprint.rs:74-151)."""
-        sc = self.pipeline.scenes[self.scene]
+    def surface_footprint(self, stream=None) -> dict:
+        """Source-chart footprint of every pixel from two adjoint passes (seed
+        r, then phi_s; gl_vjp_surface_footprint): the analytic counterpart of
+        TracePipeline.surface_footprint's +-0.25-pixel central differences
+        (surface_footprint_v1, footprint.rs:74-151), same result layout."""
         n = self.target.rays
-        ones = torch.ones(n, dtype=torch.float32, device=self.target.status.device)
-        _, dr = self.backward_observables(grad_r=ones, direction=True)
-        _, dphi = self.backward_observables(grad_phi=ones, direction=True)
-        th = math.tan(0.5 * math.radians(sc.fov_deg))
-        dsx = 2.0 * (sc.width / sc.height) * th / sc.width   # d s_x / d pixel x
-        dsy = -2.0 * th / sc.height                          # d s_y / d pixel y
-        r = self.target.observable[:n].double().cpu().numpy()
-        dr, dphi = dr.double().cpu().numpy(), dphi.double().cpu().numpy()
-        J = np.zeros((n, 2, 2))
-        J[:, 0, 0], J[:, 0, 1] = dr[0] * dsx, dr[1] * dsy
-        J[:, 1, 0], J[:, 1, 1] = r * dphi[0] * dsx, r * dphi[1] * dsy
-        return J
+        dev = self.target.status.device
+        jac = torch.zeros(4 * n, dtype=torch.float32, device=dev)
+        meta = torch.zeros(n, dtype=torch.int32, device=dev)
+        out = self.target.c_struct()
+        _capi.check(self.pipeline._lib.gl_vjp_surface_footprint(
+            self._handle, C.byref(out), C.c_void_p(jac.data_ptr()), C.c_void_p(meta.data_ptr()),
+            _stream_handle(stream)))
+        torch.cuda.synchronize()
+        return _footprint_result(jac, meta, n)
+
+    def lensing_jacobian(self) -> np.ndarray:
+        """Per-pixel d(r, r phi_s)/d(pixel x, y), shape (n, 2, 2), zero where
+        the footprint does not resolve."""
+        return self.surface_footprint()["J"]
 
     def close(self) -> None:
         if self._handle:

@@ -364,6 +364,19 @@ gl_status gl_vjp_backward(gl_vjp_state* state, const float* grad_g, double* grad
 gl_status gl_vjp_backward_observables(gl_vjp_state* state, const float* grad_r,
                                       const float* grad_phi, const float* grad_g,
                                       double* grad_params, float* grad_direction, void* stream);
+/* The source-chart footprint of every pixel of the state's scene from two
+ * backward passes (seed dL/dr = 1, then dL/dphi_s = 1): the analytic form of
+ * gl_trace_surface_footprint (surface_footprint_v1,
+ * crates/gravlume-reference/src/footprint.rs:74-151) without its four
+ * neighbour rays. `out` holds the forward's planes (status and observable
+ * are read). jacobian: device float[4 * n_rays], (dr/dx, dr/dy,
+ * r dphi/dx, r dphi/dy) in M per pixel (x right, y down), zeros unless
+ * resolved; meta: device uint32[n_rays] as in gl_trace_surface_footprint,
+ * resolved when the ray ended unambiguously on the surface (no flags,
+ * candidate mask exactly the surface bit) within the adjoint's step
+ * capacity. Enqueued on `stream`, no synchronization. */
+gl_status gl_vjp_surface_footprint(gl_vjp_state* state, const gl_ray_outputs* out,
+                                   float* jacobian, uint32_t* meta, void* stream);
 gl_status gl_vjp_destroy(gl_vjp_state* state);
 
 /* Per-thread text of the last error. */

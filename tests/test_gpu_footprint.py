@@ -64,12 +64,13 @@ def test_surface_footprint_matches_the_oracle(gl, oracle):
     assert not pipe.surface_footprint(0, off)["resolved"].any()
 
 
-def test_lensing_jacobian_from_the_adjoint_matches_the_oracle(gl, oracle):
-    """The analytic source-chart Jacobian from two discrete-adjoint passes
-    (seed r, then phi_s; VjpTrace.lensing_jacobian) against f64 central
-    differences of the oracle at +-0.02 pixel and rtol 1e-12, on the fixture
-    pixel and 48 seeded surface pixels of the default view, within the
-    reference's 3e-3 max-norm gate."""
+def test_adjoint_footprint_matches_the_oracle(gl, oracle):
+    """The analytic footprint from two discrete-adjoint passes (seed r, then
+    phi_s; gl_vjp_surface_footprint) against f64 central differences of the
+    oracle at +-0.02 pixel and rtol 1e-12, on the fixture pixel and 48 seeded
+    resolved pixels of the default view, within the reference's 3e-3
+    max-norm gate; parity and singular values as the oracle's, and the
+    five-ray GPU footprint on the same pixels within the same gate."""
     from gravlume_amd import scene as S
     import torch
     sc = S.Scene(spin=0.8, charge=0.0, inclination_deg=60.0, observer_radius=30.0, width=1280,
@@ -78,10 +79,18 @@ def test_lensing_jacobian_from_the_adjoint_matches_the_oracle(gl, oracle):
     pipe = gl.TracePipeline([sc])
     vt = gl.VjpTrace(pipe, 0)
     torch.cuda.synchronize()
-    J = vt.lensing_jacobian()
+    fp = vt.surface_footprint()
     h = vt.target.to_host()
-    srf = np.flatnonzero((h["termination"] == 7) & (np.abs(J).sum(axis=(1, 2)) > 0))
-    pix = np.concatenate([[16 * 1280 + 640], np.random.default_rng(21).choice(srf, 48, replace=False)])
+    srf = h["termination"] == 7
+    # resolved only on unambiguous surface hits (and on all of them that stay
+    # within the adjoint's step capacity)
+    unamb = srf & (h["flags"] == 0) & (h["candidates"] == 4)
+    assert not (fp["resolved"] & ~unamb).any()
+    assert fp["resolved"].sum() >= 0.999 * unamb.sum()
+    assert not fp["J"][~fp["resolved"]].any()
+    pool = np.flatnonzero(fp["resolved"])
+    pix = np.concatenate([[16 * 1280 + 640], np.random.default_rng(21).choice(pool, 48, replace=False)])
+    assert fp["resolved"][pix[0]]
     obs = oracle.observation(1.0, sc.spin, sc.charge, float(sc.chart), [0.0, *sc.observer_xyz()],
                              [0, 0, 0, 0], [0, 0, 1], 1.0, sc.width, sc.height,
                              math.radians(sc.fov_deg))
@@ -90,12 +99,17 @@ def test_lensing_jacobian_from_the_adjoint_matches_the_oracle(gl, oracle):
                                    pix % sc.width, pix // sc.width, offset=0.02)
     ok = cpu["status"] == 0
     assert ok[0] and ok.sum() >= 40
-    Jv = J[pix]
+    Jv = fp["J"][pix]
     err = np.max(np.abs(Jv - cpu["J"]), axis=(1, 2)) / np.max(np.abs(cpu["J"]), axis=(1, 2))
-    print("adjoint Jacobian max-norm relative error: fixture", err[0], "worst", err[ok].max(),
+    print("adjoint footprint max-norm relative error: fixture", err[0], "worst", err[ok].max(),
           "median", np.median(err[ok]))
     assert np.all(err[ok] <= GATE)
-    det_v = Jv[:, 0, 0] * Jv[:, 1, 1] - Jv[:, 0, 1] * Jv[:, 1, 0]
-    det_c = cpu["J"][:, 0, 0] * cpu["J"][:, 1, 1] - cpu["J"][:, 0, 1] * cpu["J"][:, 1, 0]
-    assert np.all(np.sign(det_v[ok]) == np.sign(det_c[ok]))
+    assert np.array_equal(fp["parity"][pix][ok], cpu["parity"][ok])
+    assert np.allclose(fp["singular"][pix][ok], cpu["singular"][ok], rtol=5e-3, atol=1e-6)
+    fd = pipe.surface_footprint(0, pix)
+    both = fd["resolved"] & ok
+    assert both.sum() >= 40
+    efd = np.max(np.abs(fd["J"][both] - Jv[both]), axis=(1, 2)) / np.max(np.abs(Jv[both]), axis=(1, 2))
+    print("five-ray GPU footprint vs adjoint: worst", efd.max())
+    assert np.all(efd <= GATE)
     vt.close()
