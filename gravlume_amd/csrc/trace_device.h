@@ -806,8 +806,11 @@ GL_HD void step(const SceneT<T>& sc, RayT<T>& R, T omega, bool reproject = false
     thz = z_root<T, T>(K, h, z0, z1);
     T xs, ysv, zs;
     dense_position<T>(K, h, y0[0], y0[1], y0[2], thz, xs, ysv, zs);
-    Geom<T> gs = geometry<T>(xs, ysv, zs, sc.mass, sc.spin, sc.spin2, sc.charge2, sc.chart);
-    if (gs.r >= sc.disk_in && gs.r <= sc.disk_out) fr[2] = thz;
+    // On the equatorial plane the oblate radius has the closed form
+    // r^2 = x^2 + y^2 - a^2 (the quartic's root at z = 0); the disk test
+    // compares squares, and a crossing inside the ring (r^2 < 0) fails it.
+    const T r2s = xs * xs + ysv * ysv - sc.spin2;
+    if (r2s >= sc.disk_in * sc.disk_in && r2s <= sc.disk_out * sc.disk_out) fr[2] = thz;
   }
   if ((R.armed & kEvEsc) && r0 < sc.r_escape && r1 >= sc.r_escape)
     fr[3] = radius_root<T>(sc, K, h, y0[0], y0[1], y0[2], r0 - sc.r_escape, r1 - sc.r_escape,
