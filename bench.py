@@ -500,6 +500,16 @@ def main():
             pinned[k].copy_(getattr(target, k), non_blocking=True)
     torch.cuda.synchronize()
     t_pcie = time.perf_counter() - t0
+    # the same persistent frame captured once as a hipGraph and replayed
+    graph = pipe.frame_graph([target])
+    graph.launch()
+    graph.stream.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        graph.launch()
+    graph.stream.synchronize()
+    t_graph = time.perf_counter() - t0
+    torch.cuda.synchronize()
     t_plain, k_plain, f_plain = timed(lambda: pipe.encode_batch(plain_target))
     host = target.to_host()
     plain_host = plain_target.to_host()
@@ -531,6 +541,7 @@ def main():
                    "rays_per_step": rays, "kernel": "k_persistent"},
         "plain_rays_per_s": plain_value,
         "host_inclusive_rays_per_s": rays * args.steps / t_pcie,
+        "graph_rays_per_s": rays * args.steps / t_graph,
         "persistent_over_plain": value / plain_value,
         "kernel_ms": {"persistent": k_pers, "persistent_f64_finisher": f_pers, "plain": k_plain,
                       "plain_f64_finisher": f_plain},
