@@ -550,8 +550,19 @@ GL_HD U z_root(const T K[7][7], T h, T z0, T z1) {
   }
   U hh = U(h), zz0 = U(z0);
   for (int j = 0; j < 4; ++j) c[j] *= hh;
-  U lo = U(0), hi = U(1), flo = zz0, fhi = U(z1);
-  U th = flo / (flo - fhi);
+  // the derivative's coefficients, and the quotients: the f32 trace divides
+  // through the hardware reciprocal (1 ulp) as everywhere else in its step,
+  // the f64 disk root keeps the correctly rounded division
+  const U dc[3] = {U(2) * c[1], U(3) * c[2], U(4) * c[3]};
+  auto quot = [](U a, U b) -> U {
+    if constexpr (sizeof(U) >= sizeof(double)) {
+      return a / b;
+    } else {
+      return a * gl_rcp<U>(b);
+    }
+  };
+  U lo = U(0), hi = U(1), flo = zz0;
+  U th = quot(flo, flo - U(z1));
   if (!(th > lo && th < hi)) th = U(0.5);
   // Bracket width that ends the iteration: 1e-13 in f64; in f32 two ulps of
   // a fraction near 1 (2^-22). A 1e-13 bracket lies below binary32
@@ -562,16 +573,15 @@ GL_HD U z_root(const T K[7][7], T h, T z0, T z1) {
   const U stop = sizeof(U) >= sizeof(double) ? U(1e-13) : U(2.384185791015625e-7);
   for (int it = 0; it < 40; ++it) {
     U f = zz0 + th * (c[0] + th * (c[1] + th * (c[2] + th * c[3])));
-    U df = c[0] + th * (U(2) * c[1] + th * (U(3) * c[2] + th * U(4) * c[3]));
+    U df = c[0] + th * (dc[0] + th * (dc[1] + th * dc[2]));
     if (f == U(0)) return th;
     if ((f < U(0)) == (flo < U(0))) {
       lo = th;
       flo = f;
     } else {
       hi = th;
-      fhi = f;
     }
-    U nt = th - f / df;
+    U nt = th - quot(f, df);
     th = (nt > lo && nt < hi) ? nt : U(0.5) * (lo + hi);
     if (hi - lo <= stop) break;
   }
