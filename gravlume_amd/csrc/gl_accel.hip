@@ -338,8 +338,10 @@ __global__ void __launch_bounds__(64) k_surface_footprint(const SceneDev* __rest
     const uint32_t e = base + group;
     float r = 0.0f, phi = 0.0f;
     uint32_t key = 0u, ok = 0u, term = 0u, flags = 0u;
-    if (e < n && k < 5u) {
-      const uint32_t pix = pixels ? pixels[e] : e;
+    // an index outside the image is not traced: zeros, meta 0
+    const uint32_t pix = e < n ? (pixels ? pixels[e] : e) : 0u;
+    const bool inside = pix < s.width * s.height;
+    if (e < n && k < 5u && inside) {
       const uint32_t px = pix % s.width, py = pix / s.width;
       RayT<float> R;
       init_image_ray_at(s, sc, px, py, s.sub_x + kOff[k][0], s.sub_y + kOff[k][1], R);

@@ -269,7 +269,8 @@ gl_status gl_trace_plan_shadow_edges(gl_trace_plan* plan, uint64_t* count, void*
  * (dr/dx, dr/dy, d(r_c dphi)/dx, d(r_c dphi)/dy) in M per pixel by central
  * differences, else zeros. meta[i]: bit 0 resolved, bits 1-2 parity (1
  * positive, 2 negative, 3 degenerate), bits 8-11 the centre's termination,
- * bits 16-23 its flags. Device outputs. */
+ * bits 16-23 its flags. An index outside the image is not traced (zeros,
+ * meta 0). Device outputs. */
 gl_status gl_trace_surface_footprint(gl_trace_plan* plan, uint32_t scene, const uint32_t* pixels,
                                      size_t n, float* jacobian, uint32_t* meta, void* stream);
 
