@@ -134,7 +134,8 @@ GL_HD Geom<T> geometry(T x, T y, T z, T mass, T spin, T spin2, T charge2, T s) {
   T disc = b * b + T(4) * az2;
   T sinv = gl_rsqrt<T>(disc);
   T S = disc * sinv;  // (r^4 + a^2 z^2) / r^2
-  T r2 = b >= T(0) ? T(0.5) * (b + S) : (T(2) * az2) * gl_rcp<T>(S - b);
+  T r2 = T(0.5) * (b + S);
+  if (__builtin_expect(b < T(0), 0)) r2 = (T(2) * az2) * gl_rcp<T>(S - b);
   T rinv = gl_rsqrt<T>(r2);
   T r = r2 * rinv;
   T kinv = gl_rcp<T>(r2 + spin2);
@@ -217,7 +218,8 @@ GL_HD bool rhs_only(T x, T y, T z, T px, T py, T pz, T E, T mass, T spin, T spin
   const T disc = b * b + T(4) * az2;
   const T sinv = gl_rsqrt<T>(disc);
   const T S = disc * sinv;
-  const T r2 = b >= T(0) ? T(0.5) * (b + S) : (T(2) * az2) * gl_rcp<T>(S - b);
+  T r2 = T(0.5) * (b + S);
+  if (__builtin_expect(b < T(0), 0)) r2 = (T(2) * az2) * gl_rcp<T>(S - b);
   const T rinv = gl_rsqrt<T>(r2);
   const T r = r2 * rinv;
   const T D = r2 + spin2;
