@@ -46,7 +46,8 @@ struct gl_trace_plan {
   unsigned int* d_defer_count = nullptr;
   uint32_t defer_capacity = 0;
   int cu_count = 256;
-  gl_persistent_config pcfg = {256, 2, 8, 16, 2};
+  // whole-tile mode: the faster one on every workload measured
+  gl_persistent_config pcfg = {256, 2, 8, 64, 2};
   bool timing = false;
   bool capturing = false;
   // Timed launches: a ring of event quadruples (main start/stop, finisher

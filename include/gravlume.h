@@ -274,7 +274,8 @@ gl_status gl_trace_plan_shadow_edges(gl_trace_plan* plan, uint64_t* count, void*
 gl_status gl_trace_surface_footprint(gl_trace_plan* plan, uint32_t scene, const uint32_t* pixels,
                                      size_t n, float* jacobian, uint32_t* meta, void* stream);
 
-/* Launch parameters of the persistent kernel (set by the autotuner). */
+/* Launch parameters of the persistent kernel (set by the autotuner; a new
+ * plan starts in whole-tile mode, {256, 2, 8, 64, 2}). */
 typedef struct gl_persistent_config {
   uint32_t block_size;       /* lanes per workgroup: 64, 128, 192 or 256 */
   uint32_t blocks_per_cu;    /* resident workgroups per CU (capped by occupancy) */

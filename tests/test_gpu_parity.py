@@ -228,6 +228,8 @@ def test_multi_scene_persistent_matches_single_scene_launches(gl):
         s.width = s.height = 64
     scenes[1].width = 67  # a ragged viewport among them
     pipe = gl.TracePipeline(scenes)
+    pipe.set_persistent_config(block_size=256, blocks_per_cu=2, refill_interval=8,
+                               refill_threshold=16, waves_per_simd=2)  # lane refill
     multi = [pipe.create_target(i) for i in range(len(scenes))]
     pipe.encode_persistent(multi)
     single = []
