@@ -237,7 +237,9 @@ class TracePipeline:
         return _footprint_result(jac, meta, n)
 
     def set_persistent_config(self, block_size=256, blocks_per_cu=2, refill_interval=8,
-                              refill_threshold=16, waves_per_simd=2) -> None:
+                              refill_threshold=64, waves_per_simd=2) -> None:
+        """Launch parameters of the persistent kernel (gl_persistent_config);
+        the defaults are a new plan's: whole-tile mode, two waves per SIMD."""
         c = _capi.PersistentConfig(block_size, blocks_per_cu, refill_interval, refill_threshold,
                                    waves_per_simd)
         _capi.check(self._lib.gl_trace_plan_set_persistent_config(self._handle, C.byref(c)))
