@@ -44,6 +44,7 @@ struct gl_trace_plan {
   bool frame_copy_pending = false;
   DeferRecord* d_defer = nullptr;
   unsigned int* d_defer_count = nullptr;
+  uint32_t* d_tiles = nullptr;  // per-scene tile orders, then tile costs
   uint32_t defer_capacity = 0;
   int cu_count = 256;
   // whole-tile mode: the faster one on every workload measured

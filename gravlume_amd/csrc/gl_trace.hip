@@ -214,7 +214,7 @@ __global__ void __launch_bounds__(256, kWaves) k_persistent_tiles(
     SceneT<float> sc = scene_f32(s);
     hold_in_vgprs(sc);
     const uint32_t local = uint32_t(item - f.first_item);
-    uint32_t it = 0, idx = 0;
+    uint32_t it = 0, idx = 0, tile = 0xffffffffu;
     bool have = false;
     RayT<float> R;
     if (f.states) {  // geodesic batch: 64 consecutive requests (padded to 64)
@@ -225,8 +225,10 @@ __global__ void __launch_bounds__(256, kWaves) k_persistent_tiles(
         have = true;
       }
     } else {
+      // the queue's tile slot -> the tile it hands out this launch
+      tile = s.tile_order[local >> 6];
       uint32_t px, py;
-      image_pixel(local, s.tiles_x, px, py);
+      image_pixel((tile << 6) | (local & 63u), s.tiles_x, px, py);
       if (px < s.width && py < s.height) {
         idx = py * s.width + px;
         init_image_ray(s, sc, px, py, R);
@@ -248,6 +250,7 @@ __global__ void __launch_bounds__(256, kWaves) k_persistent_tiles(
     uint32_t longest = it;  // wave iterations of this tile: its longest lane
     for (int off = 32; off > 0; off >>= 1) longest = max(longest, (uint32_t)__shfl_xor(longest, off));
     wave_steps += longest > 0u ? longest - 1u : 0u;
+    if (lane == 0 && tile != 0xffffffffu) s.tile_cost[tile] = longest;
   }
   for (int off = 32; off > 0; off >>= 1) lane_attempts += __shfl_xor(lane_attempts, off);
   if (lane == 0) {
@@ -255,6 +258,47 @@ __global__ void __launch_bounds__(256, kWaves) k_persistent_tiles(
     atomicAdd(&queue->slots, 64ull * wave_steps);
     atomicAdd(&queue->refills, (unsigned long long)tiles);
   }
+}
+
+// The next whole-tile launch's tile order for each image scene of a frame:
+// the tiles by decreasing cost of this launch (wave iterations of the tile's
+// longest lane), so the queue hands out the longest tiles first and the
+// launch ends on short ones (longest-processing-time-first). A counting sort
+// over 256 logarithmic cost buckets (16 per octave), one workgroup per
+// scene; the order inside a bucket follows LDS atomics. Which wavefront
+// traces a tile, and when, leaves every output unchanged.
+__device__ __forceinline__ uint32_t cost_bucket(uint32_t cost) {
+  if (cost == 0u) return 255u;
+  const uint32_t lg = 31u - uint32_t(__clz(cost));
+  const uint32_t frac = lg >= 4u ? (cost >> (lg - 4u)) & 15u : (cost << (4u - lg)) & 15u;
+  return 255u - min(255u, lg * 16u + frac);
+}
+
+__global__ void __launch_bounds__(1024) k_tile_order(const SceneDev* __restrict__ scenes,
+                                                     const FrameScene* __restrict__ frame) {
+  __shared__ uint32_t hist[256];
+  const FrameScene& f = frame[blockIdx.x];
+  if (f.states) return;
+  for (uint32_t b = 0; b < blockIdx.x; ++b)  // a scene listed twice is sorted once
+    if (frame[b].scene == f.scene && !frame[b].states) return;
+  const SceneDev& s = scenes[f.scene];
+  const uint32_t n = s.tiles_x * ((s.height + 7u) / 8u);
+  uint32_t* order = const_cast<uint32_t*>(s.tile_order);
+  for (uint32_t i = threadIdx.x; i < 256u; i += blockDim.x) hist[i] = 0u;
+  __syncthreads();
+  for (uint32_t t = threadIdx.x; t < n; t += blockDim.x) atomicAdd(&hist[cost_bucket(s.tile_cost[t])], 1u);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t sum = 0u;
+    for (uint32_t i = 0; i < 256u; ++i) {
+      const uint32_t c = hist[i];
+      hist[i] = sum;
+      sum += c;
+    }
+  }
+  __syncthreads();
+  for (uint32_t t = threadIdx.x; t < n; t += blockDim.x)
+    order[atomicAdd(&hist[cost_bucket(s.tile_cost[t])], 1u)] = t;
 }
 
 __global__ void __launch_bounds__(64) k_finish_f64(const SceneDev* __restrict__ scenes,
@@ -718,6 +762,12 @@ gl_status launch_persistent(gl_trace_plan* p, const FrameScene* d_table, size_t 
     GL_HIP(hipGetLastError());
     mark(p, 3, st);
   }
+  if (tiles) {
+    // the next whole-tile launch hands out this launch's longest tiles first
+    hipLaunchKernelGGL(k_tile_order, dim3(uint32_t(n_table)), dim3(1024), 0, st, p->d_scenes,
+                       d_table);
+    GL_HIP(hipGetLastError());
+  }
   return GL_OK;
 }
 
@@ -811,6 +861,30 @@ gl_status gl_trace_plan_create(const gl_scene_desc* scenes, size_t n_scenes,
       return cleanup(e, "upload band table");
     for (auto& s : p->scenes) s.lut = p->d_lut;
   }
+  {
+    // tile orders (identity to start) and tile costs of every scene
+    size_t total_tiles = 0;
+    for (const auto& s : p->scenes) total_tiles += size_t(s.tiles_x) * ((s.height + 7u) / 8u);
+    std::vector<uint32_t> init(2 * total_tiles, 0u);
+    size_t off = 0;
+    for (const auto& s : p->scenes) {
+      const size_t n = size_t(s.tiles_x) * ((s.height + 7u) / 8u);
+      for (size_t t = 0; t < n; ++t) init[off + t] = uint32_t(t);
+      off += n;
+    }
+    if ((e = hipMalloc(&p->d_tiles, std::max<size_t>(init.size(), 1) * sizeof(uint32_t))) != hipSuccess)
+      return cleanup(e, "hipMalloc tile order");
+    if (!init.empty() &&
+        (e = hipMemcpy(p->d_tiles, init.data(), init.size() * sizeof(uint32_t), hipMemcpyHostToDevice)) != hipSuccess)
+      return cleanup(e, "upload tile order");
+    off = 0;
+    for (auto& s : p->scenes) {
+      const size_t n = size_t(s.tiles_x) * ((s.height + 7u) / 8u);
+      s.tile_order = p->d_tiles + off;
+      s.tile_cost = p->d_tiles + total_tiles + off;
+      off += n;
+    }
+  }
   if ((e = hipMalloc(&p->d_scenes, n_scenes * sizeof(SceneDev))) != hipSuccess) return cleanup(e, "hipMalloc scenes");
   if ((e = hipMemcpy(p->d_scenes, p->scenes.data(), n_scenes * sizeof(SceneDev), hipMemcpyHostToDevice)) != hipSuccess)
     return cleanup(e, "upload scenes");
@@ -847,6 +921,7 @@ gl_status gl_trace_plan_destroy(gl_trace_plan* p) {
   if (p->h_frame) (void)hipHostFree(p->h_frame);
   if (p->d_defer) (void)hipFree(p->d_defer);
   if (p->d_defer_count) (void)hipFree(p->d_defer_count);
+  if (p->d_tiles) (void)hipFree(p->d_tiles);
   if (p->frame_copied) (void)hipEventDestroy(p->frame_copied);
   for (auto& t : p->timed)
     for (hipEvent_t e : t.ev)

@@ -52,9 +52,13 @@ struct alignas(16) SceneDev {
   const float* lut;        // 4097 x 4 log2 band fractions (blackbody scenes)
   float safety, min_factor, max_factor;  // step controller factors
   uint32_t pad_;
+  // whole-tile persistent queue: the order in which the tiles are handed out
+  // (longest of the previous launch first) and this launch's cost per tile
+  const uint32_t* tile_order;
+  uint32_t* tile_cost;
 };
 
-static_assert(sizeof(SceneDev) == 320, "SceneDev must stay 320 bytes");
+static_assert(sizeof(SceneDev) == 336, "SceneDev must stay 336 bytes");
 
 // Work-queue and statistics block of one persistent launch.
 struct alignas(64) QueueDev {

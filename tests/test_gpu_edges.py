@@ -119,3 +119,45 @@ def test_footprint_pixel_outside_the_image_is_not_traced(gl):
     ref = pipe.surface_footprint(0, inside)
     assert np.array_equal(fp["J"][[0, 2]], ref["J"])
     assert np.array_equal(fp["termination"][[0, 2]], ref["termination"])
+
+
+def test_whole_tile_launches_in_adapted_tile_order_stay_bit_identical(gl):
+    """Each whole-tile launch hands out its tiles by decreasing cost of the
+    previous launch (the order kernel after the launch): four consecutive
+    launches over a two-scene table, and over a hipGraph of the same frame,
+    all equal the plain dispatch bit for bit."""
+    from gravlume_amd import scene as S
+    import torch
+    scenes = [S.single_frame_1080p(width=480, height=270),
+              _default_view(S, width=200, height=120)]
+    pipe = gl.TracePipeline(scenes)
+    refs = []
+    for i in range(2):
+        t = pipe.create_target(i)
+        pipe.encode_batch(t)
+        refs.append(t)
+    torch.cuda.synchronize()
+    refs = [t.to_host() for t in refs]
+    pipe.set_persistent_config(block_size=256, blocks_per_cu=2, refill_interval=8,
+                               refill_threshold=64, waves_per_simd=2)
+    ts = [pipe.create_target(0), pipe.create_target(1)]
+    for launch in range(4):
+        for t in ts:
+            t.status.zero_()
+        pipe.encode_persistent(ts)
+        torch.cuda.synchronize()
+        for i in range(2):
+            h = ts[i].to_host()
+            for k in refs[i]:
+                assert np.array_equal(np.asarray(h[k]), np.asarray(refs[i][k])), (launch, i, k)
+    graph = pipe.frame_graph(ts)
+    for _ in range(3):
+        for t in ts:
+            t.status.zero_()
+        torch.cuda.synchronize()
+        graph.launch()
+        graph.stream.synchronize()
+        for i in range(2):
+            h = ts[i].to_host()
+            for k in refs[i]:
+                assert np.array_equal(np.asarray(h[k]), np.asarray(refs[i][k])), ("graph", i, k)
