@@ -201,6 +201,7 @@ __global__ void __launch_bounds__(256, kWaves) k_persistent_tiles(
   const unsigned long long total = queue->total;
   const uint32_t lane = __lane_id();
   uint32_t lane_attempts = 0, tiles = 0, wave_steps = 0;
+  bool cost_changed = false;
   for (;;) {
     unsigned long long base = 0;
     if (lane == 0) base = atomicAdd(&queue->next, 64ull);
@@ -250,13 +251,17 @@ __global__ void __launch_bounds__(256, kWaves) k_persistent_tiles(
     uint32_t longest = it;  // wave iterations of this tile: its longest lane
     for (int off = 32; off > 0; off >>= 1) longest = max(longest, (uint32_t)__shfl_xor(longest, off));
     wave_steps += longest > 0u ? longest - 1u : 0u;
-    if (lane == 0 && tile != 0xffffffffu) s.tile_cost[tile] = longest;
+    if (lane == 0 && tile != 0xffffffffu && s.tile_cost[tile] != longest) {
+      s.tile_cost[tile] = longest;
+      cost_changed = true;
+    }
   }
   for (int off = 32; off > 0; off >>= 1) lane_attempts += __shfl_xor(lane_attempts, off);
   if (lane == 0) {
     atomicAdd(&queue->attempts, (unsigned long long)lane_attempts);
     atomicAdd(&queue->slots, 64ull * wave_steps);
     atomicAdd(&queue->refills, (unsigned long long)tiles);
+    if (cost_changed) atomicOr(&queue->order_dirty, 1ull);
   }
 }
 
@@ -264,41 +269,93 @@ __global__ void __launch_bounds__(256, kWaves) k_persistent_tiles(
 // the tiles by decreasing cost of this launch (wave iterations of the tile's
 // longest lane), so the queue hands out the longest tiles first and the
 // launch ends on short ones (longest-processing-time-first). A counting sort
-// over 256 logarithmic cost buckets (16 per octave), one workgroup per
-// scene; the order inside a bucket follows LDS atomics. Which wavefront
-// traces a tile, and when, leaves every output unchanged.
+// over 16 octave buckets of the cost, one workgroup per scene: thread i takes
+// tiles i, i + 1024, ... (coalesced loads, 32 in flight) and keeps its own
+// counter per bucket in LDS (no two threads touch one counter), one
+// block-wide scan of the counters gives every thread its output offsets, and
+// a second pass writes the tiles in place; the order within a bucket is
+// fixed by (tile mod 1024, tile / 1024). Which wavefront traces a tile, and
+// when, leaves every output unchanged.
+constexpr uint32_t kOrderThreads = 1024u, kOrderBuckets = 16u;
+
 __device__ __forceinline__ uint32_t cost_bucket(uint32_t cost) {
-  if (cost == 0u) return 255u;
-  const uint32_t lg = 31u - uint32_t(__clz(cost));
-  const uint32_t frac = lg >= 4u ? (cost >> (lg - 4u)) & 15u : (cost << (4u - lg)) & 15u;
-  return 255u - min(255u, lg * 16u + frac);
+  if (cost == 0u) return kOrderBuckets - 1u;
+  const uint32_t octave = 31u - uint32_t(__clz(cost));  // floor(log2 cost)
+  return kOrderBuckets - 1u - min(kOrderBuckets - 1u, octave);
 }
 
-__global__ void __launch_bounds__(1024) k_tile_order(const SceneDev* __restrict__ scenes,
-                                                     const FrameScene* __restrict__ frame) {
-  __shared__ uint32_t hist[256];
+__global__ void __launch_bounds__(kOrderThreads) k_tile_order(const SceneDev* __restrict__ scenes,
+                                                              const FrameScene* __restrict__ frame,
+                                                              const QueueDev* __restrict__ queue) {
+  __shared__ uint32_t cnt[kOrderBuckets * kOrderThreads];  // bucket-major, 64 KiB
+  __shared__ uint32_t part[kOrderThreads];
+  // no tile of the launch changed its cost: the order stands (a static view
+  // is sorted once, after the launch that first measured it)
+  if (queue->order_dirty == 0ull) return;
   const FrameScene& f = frame[blockIdx.x];
   if (f.states) return;
   for (uint32_t b = 0; b < blockIdx.x; ++b)  // a scene listed twice is sorted once
     if (frame[b].scene == f.scene && !frame[b].states) return;
   const SceneDev& s = scenes[f.scene];
   const uint32_t n = s.tiles_x * ((s.height + 7u) / 8u);
+  const uint32_t* cost = s.tile_cost;
   uint32_t* order = const_cast<uint32_t*>(s.tile_order);
-  for (uint32_t i = threadIdx.x; i < 256u; i += blockDim.x) hist[i] = 0u;
-  __syncthreads();
-  for (uint32_t t = threadIdx.x; t < n; t += blockDim.x) atomicAdd(&hist[cost_bucket(s.tile_cost[t])], 1u);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t sum = 0u;
-    for (uint32_t i = 0; i < 256u; ++i) {
-      const uint32_t c = hist[i];
-      hist[i] = sum;
-      sum += c;
+  const uint32_t tid = threadIdx.x;
+  constexpr uint32_t kBatch = 32u;
+  const uint32_t stride = kOrderThreads * kBatch;
+#pragma unroll
+  for (uint32_t k = 0; k < kOrderBuckets; ++k) cnt[k * kOrderThreads + tid] = 0u;
+  for (uint32_t t0 = tid; t0 < n; t0 += stride) {
+    uint32_t c[kBatch];
+#pragma unroll
+    for (uint32_t j = 0; j < kBatch; ++j) {
+      const uint32_t t = t0 + j * kOrderThreads;
+      c[j] = t < n ? cost[t] : 0xffffffffu;
     }
+#pragma unroll
+    for (uint32_t j = 0; j < kBatch; ++j)
+      if (t0 + j * kOrderThreads < n) cnt[cost_bucket(c[j]) * kOrderThreads + tid] += 1u;
   }
   __syncthreads();
-  for (uint32_t t = threadIdx.x; t < n; t += blockDim.x)
-    order[atomicAdd(&hist[cost_bucket(s.tile_cost[t])], 1u)] = t;
+  // exclusive scan of cnt in bucket-major order: thread i takes entries
+  // [16 i, 16 i + 16), then the 1024 partial sums are scanned
+  uint32_t local[kOrderBuckets], sum = 0u;
+#pragma unroll
+  for (uint32_t j = 0; j < kOrderBuckets; ++j) {
+    const uint32_t v = cnt[kOrderBuckets * tid + j];
+    local[j] = sum;
+    sum += v;
+  }
+  part[tid] = sum;
+  __syncthreads();
+  for (uint32_t d = 1; d < kOrderThreads; d <<= 1) {  // inclusive scan of part
+    const uint32_t add = tid >= d ? part[tid - d] : 0u;
+    __syncthreads();
+    part[tid] += add;
+    __syncthreads();
+  }
+  const uint32_t base = part[tid] - sum;  // exclusive
+#pragma unroll
+  for (uint32_t j = 0; j < kOrderBuckets; ++j) cnt[kOrderBuckets * tid + j] = base + local[j];
+  __syncthreads();
+  for (uint32_t t0 = tid; t0 < n; t0 += stride) {
+    uint32_t c[kBatch];
+#pragma unroll
+    for (uint32_t j = 0; j < kBatch; ++j) {
+      const uint32_t t = t0 + j * kOrderThreads;
+      c[j] = t < n ? cost[t] : 0xffffffffu;
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < kBatch; ++j) {
+      const uint32_t t = t0 + j * kOrderThreads;
+      if (t < n) {
+        const uint32_t at = cost_bucket(c[j]) * kOrderThreads + tid;
+        const uint32_t pos = cnt[at];
+        cnt[at] = pos + 1u;
+        order[pos] = t;
+      }
+    }
+  }
 }
 
 __global__ void __launch_bounds__(64) k_finish_f64(const SceneDev* __restrict__ scenes,
@@ -393,6 +450,7 @@ __global__ void k_queue_reset(QueueDev* q, unsigned long long total, unsigned in
     q->refills = 0;
     q->attempts = 0;
     q->slots = 0;
+    q->order_dirty = 0;
     if (defer_count) {
       defer_count[0] = 0;  // deferred rays
       defer_count[1] = 0;  // finisher claim cursor
@@ -764,8 +822,8 @@ gl_status launch_persistent(gl_trace_plan* p, const FrameScene* d_table, size_t 
   }
   if (tiles) {
     // the next whole-tile launch hands out this launch's longest tiles first
-    hipLaunchKernelGGL(k_tile_order, dim3(uint32_t(n_table)), dim3(1024), 0, st, p->d_scenes,
-                       d_table);
+    hipLaunchKernelGGL(k_tile_order, dim3(uint32_t(n_table)), dim3(kOrderThreads), 0, st,
+                       p->d_scenes, d_table, (const QueueDev*)p->d_queue);
     GL_HIP(hipGetLastError());
   }
   return GL_OK;

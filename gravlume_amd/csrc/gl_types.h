@@ -68,7 +68,8 @@ struct alignas(64) QueueDev {
   unsigned long long attempts;  // DP5 attempts by active lanes
   unsigned long long slots;     // lane-attempt slots offered
   unsigned long long aux;       // second cursor (the accelerated sky trace's nodes)
-  unsigned long long pad[2];
+  unsigned long long order_dirty;  // a whole-tile launch saw a tile cost change
+  unsigned long long pad;
 };
 
 // Output planes as the kernels see them (see gl_ray_outputs in gravlume.h).
