@@ -436,9 +436,9 @@ def accelerated_sky_leg(g, S, torch, repeats=5):
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(rays are independent; see DESIGN.md for how the trace would shard)",
-              file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path (rays are independent, so "
+              "a frame or a work list would shard by whole scenes or tile ranges, one process per "
+              "GPU, with no collective inside the trace)", file=sys.stderr)
         sys.exit(2)
     import torch
     import gravlume_amd as g
