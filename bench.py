@@ -129,6 +129,12 @@ def cpu_baseline(scene, gpu_host, seconds, accuracy_rays, seed=20261203):
         err.update(g_rel_p99=float(np.quantile(rel, 0.99)), g_rel_max=float(rel.max()),
                    source_r_abs_max=float(np.max(np.abs(g[:, 0] - A["source_radius"][srf]))))
     err["null_drift_p99"] = float(np.quantile(gpu_host["drift"][idx][:, 0], 0.99))
+    # branch key on the rays whose class agrees with the oracle's
+    same = ok & (gt == A["termination"])
+    if same.any():
+        err["branch_key_agreement"] = {
+            k: float(np.mean((gpu_host[k][idx] == A[k])[same]))
+            for k in ("radial_turnings", "equatorial_crossings", "azimuth_winding", "polar_side")}
     return base, err
 
 
