@@ -563,6 +563,10 @@ def main():
             "not_representable": int(np.sum((host["termination"] == 7) & (host["radiance"][:, 0] < 0))),
             "model": f"{scene.emission} T_6 = {scene.six_temperature():.6g} K, I_6 = {scene.intensity_at_six}, vacuum",
         },
+        "seeds": {"escape_map": 20261203, "cpu_baseline_and_accuracy_sample": 20261203,
+                  "uneven_scenes": 20261206, "uneven_shadow_edge_rays": "20261207 + scene index",
+                  "near_critical_offsets": 20261208, "near_critical_oracle_sample": 20261209,
+                  "footprint_and_vjp": "none (every pixel of fixed scenes)"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "frac_of_nonpacked_fp32": achieved / 78.6,
                      "traffic": measured_traffic(),
