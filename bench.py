@@ -71,6 +71,16 @@ def angle(a, b):
     return np.arctan2(cr, np.sum(a * b, axis=-1))
 
 
+def _cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(scene, gpu_host, seconds, accuracy_rays, seed=20261203):
     """The oracle (CPU f64 DP5 restatement of the reference trace) on the host
     cores: a timed sample at the workload's tolerance, and a tight-tolerance
@@ -96,7 +106,23 @@ def cpu_baseline(scene, gpu_host, seconds, accuracy_rays, seed=20261203):
     elapsed = time.perf_counter() - t0
     base = {"value": done / elapsed, "unit": "rays/s", "cores": cores, "kind": "port",
             "sample": f"{done} seeded pixels of the 1920x1080 frame, f64 DP5 rtol {scene.rtol:g} "
-                      f"atol {scene.atol:g}, {elapsed:.1f} s on {cores} threads"}
+                      f"atol {scene.atol:g}, {elapsed:.1f} s on {cores} threads",
+            "host": {"nproc": os.cpu_count(), "cpu_model": _cpu_model()}}
+    # the small CPU configuration in full: 64x64, a = 0.5, Q = 0.3, rtol 1e-10
+    from gravlume_amd import scene as S
+    small = S.small_cpu_scene()
+    sobs = O.observation(1.0, small.spin, small.charge, float(small.chart),
+                         [0.0, *small.observer_xyz()], [0, 0, 0, 0], [0, 0, 1], 1.0, small.width,
+                         small.height, np.radians(small.fov_deg))
+    sev = O.events(small.escape_radius, small.disk_radii(), horizon_offset=small.horizon_offset)
+    allpix = np.arange(small.rays)
+    t1 = time.perf_counter()
+    O.observation_trace_ordered(sobs, O.config_policy(1e-10, 1e-12), sev,
+                                O.sight_planes(sobs, allpix % small.width, allpix // small.width),
+                                em, threads=cores)
+    base["small_config_full"] = {"rays_per_s": small.rays / (time.perf_counter() - t1),
+                                 "sample": "all 4096 pixels of the 64x64 a=0.5 Q=0.3 view, f64 "
+                                           "DP5 rtol 1e-10 atol 1e-12"}
     # accuracy leg: f64 reference at rtol 1e-10, and its 1/16 refinement for
     # the near-critical flag (termination changes between the two)
     idx = rng.choice(scene.rays, accuracy_rays, replace=False)
@@ -236,8 +262,13 @@ def near_critical_leg(g, S, torch, n=1_000_000, sample=1024, seed=20261208):
                                                dirs, threads=cores))
     fine = O.outcomes_to_arrays(O.trace_ordered(st, O.config_policy(1e-12 / 16, 1e-14 / 16), ev,
                                                 states[idx], dirs, threads=cores))
+    t0 = time.perf_counter()
     work = O.outcomes_to_arrays(O.trace_ordered(st, O.config_policy(sc.rtol, sc.atol), ev,
                                                 states[idx], dirs, threads=cores))
+    t_work = time.perf_counter() - t0
+    res["cpu_baseline_sample"] = {"rays_per_s": sample / t_work, "cores": cores, "kind": "port",
+                                  "sample": f"{sample} seeded requests of the batch, f64 DP5 rtol "
+                                            f"{sc.rtol:g} atol {sc.atol:g}"}
     ok = ref["termination"] == fine["termination"]
     settled = work["termination"] == ref["termination"]
     bands = {}
