@@ -4,15 +4,15 @@ integration (oracle/tangent.cpp) (needs an MI355X).
 
 Workload: 512x512, a = 0.9, Q = 0.2, inclination 70 deg, r_obs = 100 M, FOV
 20 deg, thin disk [r_ISCO, 25 M], f32 forward at rtol 1e-6, checkpoints every
-32 accepted steps. 192 seeded surface-hit pixels (away from the disk edges)
-are differentiated by the oracle at rtol 1e-12 with h = 1e-5 in a, Q,
+32 accepted steps. 1,024 seeded surface-hit pixels (away from the disk
+edges) are differentiated by the oracle at rtol 1e-12 with h = 1e-5 in a, Q,
 inclination and the sight-plane coordinates (s_x, s_y); pixels whose class
 changes under any perturbation are dropped.
 
 Tolerances: the aggregated parameter gradient sum_i w_i dg_i/dp (positive
 seeded weights) within 1e-4 relative of the differences; per-pixel direction
 gradients within 1e-4 relative at the median and 1e-3 at the 90th
-percentile (measured on MI355X: 3.5e-7, 2.5e-6 and 1.5e-5).
+percentile (measured on MI355X: 4.8e-6, 2.6e-6 and 1.9e-5).
 """
 import math
 
@@ -56,7 +56,7 @@ def test_vjp_matches_oracle_central_differences(gl, oracle):
     r = h["observable"][:, 0]
     ok = (h["termination"] == 7) & (r > disk[0] * 1.05) & (r < disk[1] * 0.95)
     rng = np.random.default_rng(20261207)
-    idx = np.sort(rng.choice(np.flatnonzero(ok), 192, replace=False))
+    idx = np.sort(rng.choice(np.flatnonzero(ok), 1024, replace=False))
     obs = oracle.observation(1.0, sc.spin, sc.charge, float(sc.chart), [0.0, *sc.observer_xyz()],
                              [0, 0, 0, 0], [0, 0, 1], 1.0, sc.width, sc.height,
                              math.radians(sc.fov_deg))
