@@ -89,6 +89,10 @@ FrameScene make_frame(uint32_t scene, uint32_t first, uint32_t rays, const gl_ra
 gl_status upload_frame(gl_trace_plan* p, const std::vector<FrameScene>& table, hipStream_t st);
 gl_status ensure_defer(gl_trace_plan* p, size_t rays);
 gl_status launch_finisher(gl_trace_plan* p, hipStream_t st);
+// The next whole-tile launch's tile order from this launch's tile costs, on
+// `st` after the launch.
+gl_status launch_tile_order(gl_trace_plan* p, const FrameScene* d_table, size_t n_table,
+                            hipStream_t st);
 MapView map_view(const gl_trace_plan* p);
 DeferBuf defer_buf(const gl_trace_plan* p);
 __global__ void k_queue_reset(QueueDev* q, unsigned long long total, unsigned int* defer_count);

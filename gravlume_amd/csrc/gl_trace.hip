@@ -820,12 +820,16 @@ gl_status launch_persistent(gl_trace_plan* p, const FrameScene* d_table, size_t 
     GL_HIP(hipGetLastError());
     mark(p, 3, st);
   }
-  if (tiles) {
-    // the next whole-tile launch hands out this launch's longest tiles first
-    hipLaunchKernelGGL(k_tile_order, dim3(uint32_t(n_table)), dim3(kOrderThreads), 0, st,
-                       p->d_scenes, d_table, (const QueueDev*)p->d_queue);
-    GL_HIP(hipGetLastError());
-  }
+  // the next whole-tile launch hands out this launch's longest tiles first
+  if (tiles) return launch_tile_order(p, d_table, n_table, st);
+  return GL_OK;
+}
+
+gl_status launch_tile_order(gl_trace_plan* p, const FrameScene* d_table, size_t n_table,
+                            hipStream_t st) {
+  hipLaunchKernelGGL(k_tile_order, dim3(uint32_t(n_table)), dim3(kOrderThreads), 0, st,
+                     p->d_scenes, d_table, (const QueueDev*)p->d_queue);
+  GL_HIP(hipGetLastError());
   return GL_OK;
 }
 
