@@ -323,6 +323,8 @@ __global__ void __launch_bounds__(64) k_shadow_refine(const SceneDev* __restrict
 // surface (flags 0, candidate mask exactly the surface bit) with one branch
 // key, and forms the central-difference Jacobian of (r, r_c dphi) against
 // the pixel coordinates, with its parity.
+constexpr uint32_t kFootprintGroups = 12u;
+
 __global__ void __launch_bounds__(64) k_surface_footprint(const SceneDev* __restrict__ scenes,
                                                           const FrameScene* __restrict__ frame,
                                                           const uint32_t* __restrict__ pixels,
@@ -333,15 +335,21 @@ __global__ void __launch_bounds__(64) k_surface_footprint(const SceneDev* __rest
   SceneT<float> sc = scene_f32(s);
   hold_in_vgprs(sc);
   const float kOff[5][2] = {{0.0f, 0.0f}, {-0.25f, 0.0f}, {0.25f, 0.0f}, {0.0f, -0.25f}, {0.0f, 0.25f}};
-  const uint32_t group = threadIdx.x >> 3, k = threadIdx.x & 7u, lead = threadIdx.x & ~7u;
-  for (uint32_t base = blockIdx.x * 8u; base < n; base += gridDim.x * 8u) {
-    const uint32_t e = base + group;
+  // twelve five-lane groups per wavefront (lanes 60-63 idle): the centre and
+  // its four neighbours of one pixel per group, gathered at the group's
+  // first lane
+  const uint32_t group = threadIdx.x / 5u, k = threadIdx.x % 5u;
+  const uint32_t lead = min(group, kFootprintGroups - 1u) * 5u;
+  const bool in_group = group < kFootprintGroups;
+  for (uint32_t base = blockIdx.x * kFootprintGroups; base < n;
+       base += gridDim.x * kFootprintGroups) {
+    const uint32_t e = in_group ? base + group : n;
     float r = 0.0f, phi = 0.0f;
     uint32_t key = 0u, ok = 0u, term = 0u, flags = 0u;
     // an index outside the image is not traced: zeros, meta 0
     const uint32_t pix = e < n ? (pixels ? pixels[e] : e) : 0u;
     const bool inside = pix < s.width * s.height;
-    if (e < n && k < 5u && inside) {
+    if (e < n && inside) {
       const uint32_t px = pix % s.width, py = pix / s.width;
       RayT<float> R;
       init_image_ray_at(s, sc, px, py, s.sub_x + kOff[k][0], s.sub_y + kOff[k][1], R);
@@ -358,7 +366,7 @@ __global__ void __launch_bounds__(64) k_surface_footprint(const SceneDev* __rest
         ok = a.ok ? 1u : 0u;
       }
     }
-    // gather the five samples at the group leader
+    // gather the five samples at the group's first lane
     float rs[5], ps[5];
     uint32_t keys[5], oks = 0u;
 #pragma unroll
@@ -525,7 +533,8 @@ gl_status gl_trace_surface_footprint(gl_trace_plan* p, uint32_t scene, const uin
   none.count = rays;
   gl_status s = upload_frame(p, {make_frame(scene, 0, uint32_t(rays), none)}, st);
   if (s) return s;
-  const uint32_t blocks = uint32_t(std::min<size_t>((n + 7) / 8, size_t(std::max(1, p->cu_count)) * 16));
+  const uint32_t blocks = uint32_t(std::min<size_t>((n + kFootprintGroups - 1) / kFootprintGroups,
+                                                    size_t(std::max(1, p->cu_count)) * 16));
   plan_mark(p, 0, st);
   hipLaunchKernelGGL(k_surface_footprint, dim3(blocks), dim3(64), 0, st, p->d_scenes, p->d_frame,
                      pixels, uint32_t(n), jacobian, meta);
