@@ -274,6 +274,39 @@ def test_uneven_scenes_match_the_oracle(gl, oracle):
         _compare(h, A, idx)
 
 
+@pytest.mark.parametrize("charge,chart,radius,inclination", [
+    (0.0, 1, 30.0, 60.0), (0.8, -1, 30.0, 35.0), (0.99, 1, 12.0, 80.0)])
+def test_static_observer_shadow_is_synges_disc(gl, charge, chart, radius, inclination):
+    """Schwarzschild and Reissner-Nordstrom frames (a = 0, 512x512, FOV 40
+    deg, no surface): every pixel centre more than 2e-4 relative from Synge's
+    shadow angle falls into the horizon inside it and escapes outside it, in
+    the plain and the whole-tile dispatch."""
+    from gravlume_amd import scene as S
+    from test_oracle_closed_forms import synge_shadow_angle
+    import torch
+    sc = S.Scene(spin=0.0, charge=charge, inclination_deg=inclination, observer_radius=radius,
+                 width=512, height=512, fov_deg=40.0, chart=chart, disk=None, escape_radius=1000.0,
+                 escape_map=False)
+    alpha = synge_shadow_angle(charge, radius)
+    pipe = gl.TracePipeline([sc])
+    a, b = pipe.create_target(0), pipe.create_target(0)
+    pipe.encode_batch(a)
+    pipe.encode_persistent([b])
+    torch.cuda.synchronize()
+    t = math.tan(math.radians(20.0))
+    idx = np.arange(sc.rays)
+    sx = t * (2.0 * (idx % 512 + 0.5) / 512 - 1.0)
+    sy = t * (1.0 - 2.0 * (idx // 512 + 0.5) / 512)
+    rel = np.arctan(np.hypot(sx, sy)) / alpha - 1.0
+    inside, outside = rel < -2e-4, rel > 2e-4
+    assert inside.sum() > 10000 and outside.sum() > 10000
+    for h in (a.to_host(), b.to_host()):
+        assert np.all(h["termination"][inside] == 1)
+        assert np.all(h["termination"][outside] == 2)
+        near = np.abs(rel) <= 2e-4
+        assert np.all(np.isin(h["termination"][near], (1, 2)))
+
+
 def test_geodesic_batch_reproduces_the_b6_fixture(gl):
     """GeodesicBatch on the GPU: the Schwarzschild b = 6 scatter
     (fixtures/v1/schwarzschild-scatter-b6.toml) escapes after one turning."""

@@ -206,3 +206,40 @@ def test_bpt_isco_and_special_cases(oracle):
     assert abs(oracle.isco_prograde(0.0, 1.0) - 4.0) <= 1e-12  # extremal RN
     assert abs(oracle.isco_prograde(0.94, 0.0) - 2.0235931047) <= 1e-9
     assert abs(oracle.isco_prograde(0.8, 0.0) - 2.9066438545) <= 1e-9
+
+
+def synge_shadow_angle(charge: float, observer_radius: float) -> float:
+    """Angular radius of the shadow seen by a static observer in Reissner-
+    Nordstrom (M = 1), Synge's formula: sin(alpha) = b_c sqrt(f(r_o)) / r_o with
+    f(r) = 1 - 2/r + Q^2/r^2, photon sphere r_ph = (3 + sqrt(9 - 8 Q^2)) / 2 and
+    critical impact parameter b_c = r_ph / sqrt(f(r_ph))."""
+    q2 = charge * charge
+    rph = 0.5 * (3.0 + math.sqrt(9.0 - 8.0 * q2))
+    f = lambda r: 1.0 - 2.0 / r + q2 / (r * r)
+    return math.asin(rph / math.sqrt(f(rph)) * math.sqrt(f(observer_radius)) / observer_radius)
+
+
+@pytest.mark.parametrize("charge,chart,radius,inclination", [
+    (0.0, 1.0, 30.0, 60.0), (0.8, -1.0, 30.0, 35.0), (0.99, 1.0, 12.0, 80.0)])
+def test_static_observer_shadow_edge_is_synges(oracle, charge, chart, radius, inclination):
+    """A pinhole view from a static observer at a = 0 (the scene camera of
+    scene.rs:211-271): rays at 1e-6 relative inside Synge's angle, in six
+    directions on the sight plane, fall into the horizon and rays 1e-6
+    outside escape, in both charts. A wrong Q^2 sign or a dropped lapse
+    factor moves the edge by more than 1e-2."""
+    O = oracle
+    alpha = synge_shadow_angle(charge, radius)
+    xyz = O.oblate_to_cartesian(0.0, chart, radius, math.radians(inclination), 0.0)
+    obs = O.observation(1.0, 0.0, charge, chart, [0.0, *xyz], [0, 0, 0, 0], [0, 0, 1], 1.0, 64, 64,
+                        math.radians(40.0))
+    ev = O.events(1000.0, None, horizon_offset=1e-6)
+    eps = np.array([-1e-3, -1e-6, 1e-6, 1e-3])
+    psi = np.linspace(0.0, 2.0 * math.pi, 7)[:-1]
+    E, P = np.meshgrid(eps, psi)
+    t = np.tan(alpha * (1.0 + E.ravel()))
+    sxy = np.stack([t * np.cos(P.ravel()), t * np.sin(P.ravel())], axis=-1)
+    out, st = O.observation_trace_ordered(obs, O.config_policy(1e-12, 1e-14), ev, sxy, O.Emitter(), 8)
+    assert np.all(st == 0)
+    term = O.outcomes_to_arrays(out)["termination"].reshape(E.shape)
+    assert np.all(term[:, :2] == O.TERM_HORIZON)
+    assert np.all(term[:, 2:] == O.TERM_ESCAPE)
