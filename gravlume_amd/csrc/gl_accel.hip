@@ -348,7 +348,7 @@ __global__ void __launch_bounds__(64) k_surface_footprint(const SceneDev* __rest
     uint32_t key = 0u, ok = 0u, term = 0u, flags = 0u;
     // an index outside the image is not traced: zeros, meta 0
     const uint32_t pix = e < n ? (pixels ? pixels[e] : e) : 0u;
-    const bool inside = pix < s.width * s.height;
+    const bool inside = uint64_t(pix) < uint64_t(s.width) * s.height;
     if (e < n && inside) {
       const uint32_t px = pix % s.width, py = pix / s.width;
       RayT<float> R;

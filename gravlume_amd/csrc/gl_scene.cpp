@@ -206,6 +206,10 @@ int prepare_scene(const gl_scene_desc& d, SceneDev* out, SceneHost* host, std::s
   if (d.chart != 1 && d.chart != -1) return set_err(err, "chart must be +1 or -1"), GL_ERR_INVALID_ARGUMENT;
   if (d.width == 0 || d.height == 0) return set_err(err, "viewport must be nonempty"), GL_ERR_INVALID_ARGUMENT;
   if (d.width > 65536 || d.height > 65536) return set_err(err, "viewport side above 65536"), GL_ERR_CAPACITY;
+  // pixel indices are 32-bit: the count must fit (the checked width x height
+  // product of scientific_capture.rs:294)
+  if (uint64_t(d.width) * d.height > 0xffffffffull)
+    return set_err(err, "viewport above 2^32 - 1 pixels"), GL_ERR_CAPACITY;
   if (!(d.vertical_fov > 0.0 && d.vertical_fov < M_PI))
     return set_err(err, "vertical field of view must lie in (0, pi)"), GL_ERR_INVALID_ARGUMENT;
   if (!(d.subpixel[0] >= 0.0 && d.subpixel[0] <= 1.0 && d.subpixel[1] >= 0.0 && d.subpixel[1] <= 1.0))

@@ -160,3 +160,16 @@ def test_step_controller_factors_are_validated():
         d = good.to_desc()
         setattr(d, field, value)
         assert lib.gl_scene_validate(C.byref(d)) == _capi.GL_ERR_INVALID_ARGUMENT, field
+
+
+def test_viewport_limits():
+    """Sides up to 65536 and pixel counts up to 2^32 - 1 are accepted (32-bit
+    pixel indices); 65536 x 65536 = 2^32 pixels is a capacity error, as the
+    reference's checked width x height product (scientific_capture.rs:294)."""
+    from gravlume_amd import _capi, scene as S
+    lib = _capi.load()
+    for (w, h), want in (((65536, 65535), 0), ((65535, 65536), 0), ((1, 65536), 0),
+                         ((65536, 65536), _capi.GL_ERR_CAPACITY), ((65537, 1), _capi.GL_ERR_CAPACITY),
+                         ((0, 8), _capi.GL_ERR_INVALID_ARGUMENT)):
+        d = S.small_cpu_scene(escape_map=False, width=w, height=h).to_desc()
+        assert lib.gl_scene_validate(C.byref(d)) == want, (w, h, lib.gl_last_error())
