@@ -105,7 +105,8 @@ typedef struct gl_scene_desc {
   double target_txyz[4];   /* image-centre target */
   double up_hint[3];       /* image-up hint */
   double frequency;        /* observer frequency, must be 1 (input.rs:57-63) */
-  uint32_t width, height;  /* viewport; pixel (0,0) is top-left */
+  uint32_t width, height;  /* viewport; pixel (0,0) is top-left; sides <= 65536,
+                              width * height <= 2^32 - 1 (else GL_ERR_CAPACITY) */
   double vertical_fov;     /* radians, in (0, pi) */
   double subpixel[2];      /* sample offset inside each pixel, in [0,1] */
   double escape_radius;    /* escape event r = escape_radius */
