@@ -50,6 +50,12 @@ OPT_CAPTURE_CERTIFICATE = 1
 OPT_F64_HORIZON_STEPS = 2
 OPT_F64_DISK_ROOT = 4
 OPT_ESCAPE_MAP = 8
+OPT_NEAR_CRITICAL_REFINE = 16
+
+STATUS_PROVISIONAL_SHIFT = 16
+STATUS_NEAR_CRITICAL = 1 << 20
+STATUS_REFINED = 1 << 21
+STATUS_BRANCH_UNSETTLED = 1 << 22
 
 EMISSION_BOLOMETRIC = 0
 EMISSION_BLACKBODY = 1
@@ -82,7 +88,9 @@ class SceneDesc(C.Structure):
         ("max_accepted_steps", C.c_uint32), ("max_consecutive_rejects", C.c_uint32),
         ("event_tie_tolerance", C.c_double), ("arming_band", C.c_double),
         ("drift_limit", C.c_double), ("reproject_interval", C.c_uint32),
-        ("reserved", C.c_uint32),
+        ("reserved", C.c_uint32), ("near_critical_factor", C.c_double),
+        ("branch_factor", C.c_double), ("refine_rtol", C.c_double),
+        ("refine_atol", C.c_double),
     ]
 
 
@@ -95,7 +103,9 @@ class RayOutputsC(C.Structure):
     _fields_ = [("status", C.c_void_p), ("steps", C.c_void_p), ("rejected", C.c_void_p),
                 ("observable", C.c_void_p), ("travel_time", C.c_void_p), ("affine", C.c_void_p),
                 ("drift", C.c_void_p), ("branch", C.c_void_p), ("radiance", C.c_void_p),
-                ("rgba16f", C.c_void_p), ("count", C.c_size_t)]
+                ("rgba16f", C.c_void_p), ("event_residual", C.c_void_p),
+                ("turning_radius", C.c_void_p), ("azimuth_advance", C.c_void_p),
+                ("count", C.c_size_t)]
 
 
 WORK_IMAGE = 0

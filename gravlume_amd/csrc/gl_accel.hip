@@ -50,6 +50,8 @@ __global__ void k_accel_reset(unsigned int* defer_count, uint32_t* edge_count, Q
     if (defer_count) {
       defer_count[0] = 0;
       defer_count[1] = 0;
+      defer_count[2] = 0;
+      defer_count[3] = 0;
     }
     if (edge_count) edge_count[0] = 0;
     if (q) {
@@ -86,7 +88,8 @@ __device__ __forceinline__ bool node_matches(const float4* st, int k, float u, f
 // Outputs of a pixel whose escape direction comes from the stencil: class
 // escape with candidate bit 8 and the reconstructed flag; the traced-only
 // quantities (steps, travel time, affine parameter, drifts, turnings,
-// crossings, winding) are zero.
+// crossings, winding, event residual, azimuth advance) are zero and the
+// turning radius -1.
 __device__ void write_reconstructed(const OutPlanes& o, uint32_t idx, const SceneDev& s,
                                     const MapView& map, float3 d) {
   const unsigned long long n = o.count;
@@ -106,6 +109,9 @@ __device__ void write_reconstructed(const OutPlanes& o, uint32_t idx, const Scen
     o.drift[2 * n + idx] = 0.0f;
   }
   if (o.branch) o.branch[idx] = (s.polar_side & 3u) << 24;
+  if (o.event_residual) o.event_residual[idx] = 0.0f;
+  if (o.turning_radius) o.turning_radius[idx] = -1.0f;
+  if (o.azimuth_advance) o.azimuth_advance[idx] = 0.0f;
   if (o.radiance || o.rgba16f) {
     const Rgb c = escape_texel(map, s, d.x, d.y, d.z);
     if (o.radiance) {
@@ -224,8 +230,9 @@ __global__ void __launch_bounds__(64, 2) k_sky_fused(
         }
         step<float>(sc, R, s.omega, reproject_now(sc, it), drift_now(it));
       }
-      if (!deferred) write_ray(R, fsc.out, idx, s, map);
-      // a deferred node is finished in f64 later: not a stencil node
+      // a deferred or refined node is finished in f64 later: not a stencil node
+      if (!deferred)
+        deferred = !write_ray(R, fsc.out, idx, s, map, pixel_source(px, py), refine_sink(df), 0u);
       if (node_item)
         publish_node(nodes, ready, node, deferred ? other : node_record(R, fsc.out, idx), epoch);
     } else if (node_lane) {
@@ -318,8 +325,8 @@ __global__ void __launch_bounds__(64) k_shadow_refine(const SceneDev* __restrict
 // Source-chart footprint (ObservationTracer::surface_footprint_v1,
 // crates/gravlume-reference/src/footprint.rs:74-151, and its GPU capture
 // surface_footprint_capture.wgsl:1-99): per requested pixel, a group of
-// eight lanes traces the centre and the four +-0.25-pixel neighbours (lanes
-// 0-4), the group leader checks that all five end unambiguously on the
+// five lanes (twelve groups per wavefront) traces the centre and the four
+// +-0.25-pixel neighbours, the group leader checks that all five end unambiguously on the
 // surface (flags 0, candidate mask exactly the surface bit) with one branch
 // key, and forms the central-difference Jacobian of (r, r_c dphi) against
 // the pixel coordinates, with its parity.
@@ -457,17 +464,14 @@ gl_status gl_trace_encode_accelerated(gl_trace_plan* p, uint32_t scene, const gl
     GL_HIP(hipMemsetAsync(p->d_ready, 0, p->node_capacity * sizeof(uint32_t), st));
     p->sky_epoch = 1u;
   }
-  const bool f64 = (sd->options & kOptF64Horizon) != 0;
-  if (f64) {
-    s = ensure_defer(p, size_t(sd->width) * sd->height);
-    if (s) return s;
-  }
+  const bool f64 = scene_f64_horizon(*sd), refine = scene_refine(*sd);
+  s = ensure_workspace(p->ws, size_t(sd->width) * sd->height, f64, refine);
+  if (s) return s;
   s = upload_frame(p, {make_frame(scene, 0, sd->width * sd->height, *out)}, st);
   if (s) return s;
-  hipLaunchKernelGGL(k_accel_reset, dim3(1), dim3(64), 0, st, p->d_defer_count, (uint32_t*)nullptr,
+  hipLaunchKernelGGL(k_accel_reset, dim3(1), dim3(64), 0, st, p->ws.counters, (uint32_t*)nullptr,
                      p->d_queue);
-  DeferBuf df = defer_buf(p);
-  if (!f64) df.rec = nullptr;
+  const DeferBuf df = workspace_view(p->ws, f64, refine);
   const double half_pixel = double(p->host[scene].tan_half_fov) / double(sd->height);
   const float max_err = float(std::min(half_pixel, double(kStencilMaxError)));
   const uint32_t blocks_x = (node_w + 7u) / 8u, blocks_y = (node_h + 7u) / 8u;
@@ -479,8 +483,7 @@ gl_status gl_trace_encode_accelerated(gl_trace_plan* p, uint32_t scene, const gl
                      p->sky_epoch, max_err * max_err, map_view(p), df);
   GL_HIP(hipGetLastError());
   plan_mark(p, 1, st);
-  if (f64) return launch_finisher(p, st);
-  return GL_OK;
+  return launch_f64_passes(p, p->d_frame, df, st);
 }
 
 gl_status gl_trace_refine_shadow(gl_trace_plan* p, uint32_t scene, const gl_ray_outputs* out,

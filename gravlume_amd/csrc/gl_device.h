@@ -8,6 +8,7 @@
 #include "../../include/gravlume.h"
 #include "certificate.h"
 #include "gl_types.h"
+#include "near_critical.h"
 #include "trace_device.h"
 
 namespace gl {
@@ -24,10 +25,17 @@ struct DeferRecord {
   uint32_t index;
 };
 
+// Device workspace of one launch: the deferred near-horizon rays, the
+// near-critical rays for the f64 refine pass, and their counters
+// count[0] deferred records, [1] the finisher's claim cursor, [2] refine
+// records, [3] the refine pass's claim cursor.
+struct RefineRecord;
 struct DeferBuf {
-  DeferRecord* rec;
+  DeferRecord* rec;  // nullptr: no f64 near-horizon deferral in this launch
   unsigned int* count;
   uint32_t capacity;
+  uint32_t refine_capacity;
+  RefineRecord* refine;  // nullptr: no refine pass in this launch
 };
 
 // ---------------------------------------------------------------------------
@@ -40,59 +48,80 @@ __device__ __forceinline__ void image_pixel(uint32_t item, uint32_t tiles_x, uin
   py = (tile / tiles_x) * 8u + (t >> 3);
 }
 
-// Sight momentum of pixel (px, py): direction from the tetrad, future-directed
-// p = omega (u + n), lowered with the Kerr-Schild metric at the observer
-// (scene.rs:211-271, view.rs:107-123), then projected onto the null cone in
-// f64. Out of line and returned in registers (four dwords), so the ray state
-// of the calling kernel never has its address taken and stays in VGPRs.
+// Initial state (t, x, y, z, p_t, p_x, p_y, p_z) in f64 of the backward ray
+// through viewport point (px + ox, py + oy): the sight-plane point
+// (view.rs:107-123), the direction d = (s_x e_R + s_y e_U - e_A)/|.|
+// from the f64 tetrad, p^mu = omega (u - d), lowered with the Kerr-Schild
+// metric g = eta + f l l at the observer (scene.rs:211-271).
+__device__ __forceinline__ void image_state_f64(const SceneDev& s, uint32_t px, uint32_t py,
+                                                double ox, double oy, double y[8]) {
+  const double nx = 2.0 * (double(px) + ox) / double(s.width) - 1.0;
+  const double ny = 1.0 - 2.0 * (double(py) + oy) / double(s.height);
+  const double sx = s.aspect_d * s.tan_half_fov_d * nx;
+  const double sy = s.tan_half_fov_d * ny;
+  const double inv = gl_rsqrt<double>(1.0 + sx * sx + sy * sy);
+  double d[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) d[i] = (sx * s.right_d[i] + sy * s.up_d[i] - s.arrival_d[i]) * inv;
+  const double om = double(s.omega);
+  const double pu[4] = {om * (s.u_t_d - d[0]), -om * d[1], -om * d[2], -om * d[3]};
+  const Geom<double> g = geometry<double>(s.obs_d[0], s.obs_d[1], s.obs_d[2], 1.0, s.spin_d,
+                                          s.spin_d * s.spin_d, s.charge2_d, double(s.chart));
+  // l_mu p^mu with the covariant principal null vector l_mu = (1, l)
+  const double lp = pu[0] + g.lx * pu[1] + g.ly * pu[2] + g.lz * pu[3];
+  const double fl = g.f * lp;
+  y[0] = 0.0;
+  y[1] = s.obs_d[0];
+  y[2] = s.obs_d[1];
+  y[3] = s.obs_d[2];
+  y[4] = -pu[0] + fl;
+  y[5] = pu[1] + fl * g.lx;
+  y[6] = pu[2] + fl * g.ly;
+  y[7] = pu[3] + fl * g.lz;
+}
+
+// The f32 image ray: the rounding of the f64 state above. Out of line and
+// returned in registers (four dwords), so the ray state of the calling
+// kernel never has its address taken and stays in VGPRs.
 struct Momentum {
   float px, py, pz, E;
+  float r_m;  // radius of the minimum of the ray's radial potential, 0 if untested
 };
+
+// r_m of the exact ray from f64 state y for the near-critical test
+// (near_critical.h), 0 when the scene does not test.
+__device__ __forceinline__ float critical_radius(const SceneDev& s, const double* y) {
+  if (!(s.near_critical_factor > 0.0f) || !(s.r_horizon_d > 0.0)) return 0.0f;
+  const RadialMargin<double> m = state_margin(y, s.spin_d, s.charge2_d, double(s.chart), s.r_horizon_d);
+  return m.ok ? float(m.r_m) : 0.0f;
+}
 
 __device__ __noinline__ Momentum image_momentum(const SceneDev& s, uint32_t px, uint32_t py,
                                                 float ox, float oy) {
-  const SceneT<float> sc = scene_f32(s);
-  float nx = 2.0f * (float(px) + ox) / float(s.width) - 1.0f;
-  float ny = 1.0f - 2.0f * (float(py) + oy) / float(s.height);
-  float sx = s.aspect * s.tan_half_fov * nx;
-  float sy = s.tan_half_fov * ny;
-  float inv = 1.0f / sqrtf(1.0f + sx * sx + sy * sy);
-  float d[4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i) d[i] = (sx * s.right[i] + sy * s.up[i] - s.arrival[i]) * inv;
-  // p^mu = omega (u - d), u = (u_t, 0, 0, 0)
-  float pu[4] = {s.omega * (s.u_t - d[0]), -s.omega * d[1], -s.omega * d[2], -s.omega * d[3]};
-  Geom<float> g = geometry<float>(s.obs_x, s.obs_y, s.obs_z, sc.mass, sc.spin, sc.spin2,
-                                  sc.charge2, sc.chart);
-  float lp = pu[0] + g.lx * pu[1] + g.ly * pu[2] + g.lz * pu[3];
-  float fl = g.f * lp;
-  float p_t = -pu[0] + fl;
-  Momentum m;
-  m.px = pu[1] + fl * g.lx;
-  m.py = pu[2] + fl * g.ly;
-  m.pz = pu[3] + fl * g.lz;
-  m.E = -p_t;
-  RayT<float> at;  // the observer event, for the projection
-  at.x = s.obs_x;
-  at.y = s.obs_y;
-  at.z = s.obs_z;
-  at.cx = at.cy = at.cz = 0.0f;
-  at.E = m.E;
-  reproject_null<float>(sc, at, m.px, m.py, m.pz);
-  return m;
+  double y[8];
+  image_state_f64(s, px, py, double(ox), double(oy), y);
+  return Momentum{float(y[5]), float(y[6]), float(y[7]), float(-y[4]), critical_radius(s, y)};
 }
 
-// The interval radial-capture certificate at the observer (certificate.h),
-// from the ray's constants of motion there. Out of line, once per ray.
-__device__ __noinline__ bool certify_capture(const SceneDev& s, float px, float py, float pz,
-                                             float E, float r_obs, float dr_future) {
-  const Geom<float> g = geometry<float>(s.obs_x, s.obs_y, s.obs_z, s.mass, s.spin, s.spin2,
-                                        s.charge2, s.chart);
-  const float L = s.obs_x * py - s.obs_y * px;
-  const float Q = carter<float>(g, s.obs_x, s.obs_y, s.obs_z, px, py, pz, E, s.spin2);
+__device__ __noinline__ float batch_critical_radius(const SceneDev& s, const double* y8) {
+  double y[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) y[i] = y8[i];
+  return critical_radius(s, y);
+}
+
+// The interval radial-capture certificate (certificate.h) of a backward ray
+// starting at (x, y, z), from its constants of motion there. Out of line,
+// once per ray.
+__device__ __noinline__ bool certify_capture(const SceneDev& s, float x, float y, float z,
+                                             float px, float py, float pz, float E, float r_start,
+                                             float dr_future) {
+  const Geom<float> g = geometry<float>(x, y, z, s.mass, s.spin, s.spin2, s.charge2, s.chart);
+  const float L = x * py - y * px;
+  const float Q = carter<float>(g, x, y, z, px, py, pz, E, s.spin2);
   const float q = float(sqrt(s.charge2_d));
-  return capture_certified(s.mass, s.spin, q, s.r_horizon, r_obs,
-                           s.obs_x * s.obs_x + s.obs_y * s.obs_y, dr_future, E, L, Q);
+  return capture_certified(s.mass, s.spin, q, s.r_horizon, r_start, x * x + y * y, dr_future, E,
+                           L, Q);
 }
 
 // The backward ray through point (px + ox, py + oy) of the viewport, with
@@ -101,11 +130,15 @@ __device__ __forceinline__ void init_image_ray_at(const SceneDev& s, const Scene
                                                   uint32_t px, uint32_t py, float ox, float oy,
                                                   RayT<float>& R) {
   Momentum m = image_momentum(s, px, py, ox, oy);
-  init_ray<float>(sc, R, s.obs_x, s.obs_y, s.obs_z, 0.0f, 0.0f, 0.0f, m.px, m.py, m.pz, m.E,
-                  -1.0f, s.polar_side);
+  // the observer's position with its f32 rounding as the compensation term
+  const float cx = float(double(s.obs_x) - s.obs_d[0]), cy = float(double(s.obs_y) - s.obs_d[1]),
+              cz = float(double(s.obs_z) - s.obs_d[2]);
+  init_ray<float>(sc, R, s.obs_x, s.obs_y, s.obs_z, cx, cy, cz, m.px, m.py, m.pz, m.E, -1.0f,
+                  s.polar_side);
   R.h = -s.h_init;
+  R.r_m = m.r_m;
   if ((sc.options & kOptCertificate) && R.term == kTermActive && R.flags == 0u &&
-      certify_capture(s, R.px, R.py, R.pz, R.E, R.r, R.rdot)) {
+      certify_capture(s, s.obs_x, s.obs_y, s.obs_z, R.px, R.py, R.pz, R.E, R.r, R.rdot)) {
     R.term = kTermHorizon;
     R.candidates = 2u;
     R.flags |= kFlagCertified;
@@ -126,6 +159,16 @@ __device__ void init_state_ray(const SceneDev& s, const SceneT<float>& sc, const
   init_ray<float>(sc, R, x, yy, z, cx, cy, cz, float(y[5]), float(y[6]), float(y[7]),
                   float(-y[4]), dir, side);
   R.h = dir * s.h_init;
+  R.r_m = batch_critical_radius(s, y);
+  // backward requests are pixel rays in all but their start: the same
+  // certificate applies (certificate.h: its own domain checks refuse what it
+  // cannot prove, and the ray is traced in full)
+  if ((sc.options & kOptCertificate) && dir < 0.0f && R.term == kTermActive && R.flags == 0u &&
+      certify_capture(s, x, yy, z, R.px, R.py, R.pz, R.E, R.r, R.rdot)) {
+    R.term = kTermHorizon;
+    R.candidates = 2u;
+    R.flags |= kFlagCertified;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -272,18 +315,106 @@ __device__ __forceinline__ void store_rgba16f(uint16_t* plane, uint32_t idx, Rgb
   reinterpret_cast<uint2*>(plane)[idx] = v;
 }
 
+// Where a ray started, to rebuild its exact f64 initial state at the end: a
+// geodesic-batch request (its state) or a pixel at the scene's sample offset.
+struct RaySource {
+  const double* state8;  // batch request (8 doubles), or nullptr for a pixel
+  uint32_t px, py;
+};
+
+__device__ __forceinline__ RaySource pixel_source(uint32_t px, uint32_t py) {
+  return RaySource{nullptr, px, py};
+}
+__device__ __forceinline__ RaySource batch_source(const double* state8) {
+  return RaySource{state8, 0u, 0u};
+}
+// The source of element idx of a frame slice: its batch request, or pixel
+// (idx mod width, idx / width).
+__device__ __forceinline__ RaySource frame_source(const FrameScene& f, const SceneDev& s,
+                                                  uint32_t idx) {
+  if (f.states) return batch_source(f.states + 8 * size_t(idx));
+  return pixel_source(idx % s.width, idx / s.width);
+}
+
+// Near-critical rays handed to the f64 refine pass: (frame slice, element).
+struct RefineRecord {
+  uint32_t frame_scene, index;
+};
+struct RefineSink {
+  RefineRecord* rec;  // nullptr: no refine pass in this launch
+  unsigned int* count;
+  uint32_t capacity;
+};
+__device__ __forceinline__ RefineSink refine_sink(const DeferBuf& df) {
+  return RefineSink{df.refine, df.count + 2, df.refine_capacity};
+}
+__device__ __forceinline__ RefineSink no_refine() { return RefineSink{nullptr, nullptr, 0u}; }
+
+__device__ __forceinline__ void source_state(const SceneDev& s, const double* state8, uint32_t px,
+                                             uint32_t py, double y[8]) {
+  if (state8) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) y[i] = state8[i];
+  } else {
+    image_state_f64(s, px, py, s.sub_x_d, s.sub_y_d, y);
+  }
+}
+
+struct Verdict {
+  double phi0;     // initial azimuth atan2(y, x)
+  uint32_t extra;  // GL_STATUS_NEAR_CRITICAL, GL_STATUS_BRANCH_UNSETTLED or 0
+};
+
+// The near-critical test of a finished trace (near_critical.h): from the
+// exact f64 initial state, the radial-potential margin mu and its radius
+// r_m. A ray that came within 1.25 r_m (its smallest turning radius, or its
+// terminal radius, r_near) is measured against its error budget eps, the
+// largest drift it sampled at r >= r_m and, for an f32 trace, the rounding
+// of its initial constants: with |mu| <= near_critical_factor * eps the
+// capture/scatter decision is unsettled (GL_STATUS_NEAR_CRITICAL), with
+// |mu| <= branch_factor * eps the branch key and the time and azimuth spent
+// near r_m are (GL_STATUS_BRANCH_UNSETTLED). Out of line and with scalar
+// arguments, once per ray.
+__device__ __noinline__ Verdict ray_verdict(const SceneDev& s, const double* state8, uint32_t px,
+                                            uint32_t py, float r_near, float drift,
+                                            bool exact_start) {
+  double y[8];
+  source_state(s, state8, px, py, y);
+  Verdict v;
+  v.phi0 = ::atan2(y[2], y[1]);
+  v.extra = 0u;
+  if (!(s.near_critical_factor > 0.0f) || !(s.r_horizon_d > 0.0)) return v;
+  const RadialMargin<double> m = state_margin(y, s.spin_d, s.charge2_d, double(s.chart), s.r_horizon_d);
+  if (!m.ok || !(double(r_near) <= 1.25 * m.r_m)) return v;
+  double eps = double(drift);
+  if (!exact_start) eps = fmax(eps, f32_start_error(y, s.spin_d, s.charge2_d, double(s.chart)));
+  if (!(eps == eps)) eps = 1.0;
+  const double mu = fabs(m.mu);
+  if (mu <= double(s.near_critical_factor) * eps) v.extra |= GL_STATUS_NEAR_CRITICAL;
+  if (mu <= double(s.branch_factor) * eps) v.extra |= GL_STATUS_BRANCH_UNSETTLED;
+  return v;
+}
+
+__device__ __forceinline__ bool event_class(uint32_t term) {
+  return term == kTermHorizon || term == kTermEscape || term == kTermSurface ||
+         term == kTermSingularity;
+}
+
 // Terminal observables from the localized state: the unit traversal
 // direction at an escape (geodesic.rs:480-497), or the source anchor and
 // frequency ratio of the prograde emitter in f64 at a surface hit
-// (surface.rs:75-146); then the packed status, the observable planes and the
-// radiance. Out of line (it runs once per ray and its f64 temporaries stay
-// out of the trace loop's register allocation) and called with scalars only,
-// so no ray state goes through memory.
+// (surface.rs:75-146); the event residual, the azimuth advance and the
+// turning radius; then the packed status, the observable planes and the
+// radiance. `term` is the class the trace reached (its observables are
+// written), `status` the final status word (whose class may be Uncertain).
+// Out of line (it runs once per ray and its f64 temporaries stay out of the
+// trace loop's register allocation) and called with scalars only, so no ray
+// state goes through memory.
 __device__ __noinline__ void write_terminal(const OutPlanes& o, uint32_t idx, const SceneDev& s,
-                                            MapView map, uint32_t term, uint32_t candidates,
-                                            uint32_t flags, float dir, float x, float y, float z,
-                                            float cx, float cy, float cz, float px, float py,
-                                            float pz, float E) {
+                                            MapView map, uint32_t term, uint32_t status, float dir,
+                                            float x, float y, float z, float cx, float cy,
+                                            float cz, float px, float py, float pz, float E,
+                                            int32_t winding, double phi0, float r_turn) {
   const unsigned long long n = o.count;
   float ob[3] = {0.0f, 0.0f, 0.0f};
   if (term == kTermEscape) {
@@ -291,30 +422,63 @@ __device__ __noinline__ void write_terminal(const OutPlanes& o, uint32_t idx, co
     ob[0] = v.x;
     ob[1] = v.y;
     ob[2] = v.z;
-    if (!v.ok) flags |= kFlagNonFinite;
+    if (!v.ok) status |= kFlagNonFinite << 8;
   } else if (term == kTermSurface) {
     const Anchor a = surface_anchor(s, x, y, z, cx, cy, cz, px, py, E);
     ob[0] = a.r;
     ob[1] = a.phi;
     ob[2] = a.g;
-    if (!a.ok) flags |= kFlagDenominator;
+    if (!a.ok) status |= kFlagDenominator << 8;
   }
-  if (o.status) o.status[idx] = (term & 15u) | ((candidates & 15u) << 4) | ((flags & 255u) << 8);
+  const uint32_t shown = status & 15u;  // the class the texel shows
+  if (o.status) o.status[idx] = status;
   if (o.observable) {
     o.observable[idx] = ob[0];
     o.observable[n + idx] = ob[1];
     o.observable[2 * n + idx] = ob[2];
   }
+  if (o.event_residual || o.azimuth_advance) {
+    const double xd = double(x) - double(cx), yd = double(y) - double(cy),
+                 zd = double(z) - double(cz);
+    if (o.event_residual) {
+      // (event function at the terminal - its surface) / max(1, |surface|)
+      // (geodesic_integration.wgsl:266-295)
+      double res = 0.0;
+      if (term == kTermSurface) {
+        res = zd;
+      } else if (term == kTermHorizon || term == kTermEscape || term == kTermSingularity) {
+        const Geom<double> g = geometry<double>(xd, yd, zd, 1.0, s.spin_d, s.spin_d * s.spin_d,
+                                                s.charge2_d, double(s.chart));
+        if (term == kTermSingularity) {
+          const double r2 = g.r * g.r;
+          const double guard = double(s.guard);
+          res = (r2 * r2 + s.spin_d * s.spin_d * zd * zd - guard) / fmax(1.0, guard);
+        } else {
+          const double target = term == kTermHorizon ? s.r_capture_d : double(s.r_escape);
+          res = (g.r - target) / fmax(1.0, fabs(target));
+        }
+      }
+      o.event_residual[idx] = float(res);
+    }
+    if (o.azimuth_advance) {
+      // the sum of wrapped atan2 differences between committed states equals
+      // the end azimuth minus the start azimuth plus 2 pi per branch-cut
+      // crossing counted by the winding (geodesic.rs:406-418)
+      const double adv = ::atan2(yd, xd) - phi0 + 6.283185307179586 * double(winding);
+      o.azimuth_advance[idx] = float(adv);
+    }
+  }
+  if (o.turning_radius) o.turning_radius[idx] = r_turn;
   if (o.radiance || o.rgba16f) {
     Rgb c = {0.0f, 0.0f, 0.0f};
-    float alpha = -float(term);
-    if (term == kTermEscape) {
+    float alpha = -float(shown);
+    if (shown == kTermEscape) {
       alpha = 1.0f;
       c = escape_texel(map, s, ob[0], ob[1], ob[2]);
-    } else if (term == kTermSurface) {
+    } else if (shown == kTermSurface) {
       c = ob[0] > 0.0f ? surface_radiance(s, ob[0], ob[2]) : Rgb{-1.0f, -1.0f, -1.0f};
       alpha = 2.0f;
-    } else if (term == kTermHorizon) {
+    } else if (shown == kTermHorizon) {
       alpha = 0.0f;
     }
     if (o.radiance) {
@@ -323,22 +487,50 @@ __device__ __noinline__ void write_terminal(const OutPlanes& o, uint32_t idx, co
       o.radiance[2 * n + idx] = c.b;
     }
     if (o.rgba16f) {
-      if (term == kTermSurface && c.r < 0.0f) {
+      if (shown == kTermSurface && c.r < 0.0f) {
         c = failure_colour(kTermFailure);
         alpha = -float(kTermFailure);
-      } else if (term != kTermEscape && term != kTermSurface && term != kTermHorizon) {
-        c = failure_colour(term);
+      } else if (shown != kTermEscape && shown != kTermSurface && shown != kTermHorizon) {
+        c = failure_colour(shown);
       }
       store_rgba16f(o.rgba16f, idx, c, alpha);
     }
   }
 }
 
-// All output planes of one finished ray.
-__device__ __forceinline__ void write_ray(const RayT<float>& R, const OutPlanes& o, uint32_t idx,
-                                          const SceneDev& s, const MapView& map) {
+// All output planes of one finished ray. The ray's class becomes Uncertain
+// (the class it reached kept in status bits 16-19) when its terminal events
+// tie, a turning lies inside the terminal's bracket, a drift exceeds the
+// budget (geodesic_integration.wgsl:324-330), or it is near-critical
+// (ray_verdict). A near-critical ray of a scene with the refine option goes
+// to `sink` instead, unwritten, when there is room (returns false); the f64
+// refine pass traces and writes it (refined = true: its exact f64 start
+// adds no rounding to the budget, and it is not handed on again).
+__device__ __forceinline__ bool write_ray(const RayT<float>& R, const OutPlanes& o, uint32_t idx,
+                                          const SceneDev& s, const MapView& map, RaySource src,
+                                          const RefineSink& sink, uint32_t frame_scene,
+                                          bool refined = false) {
   const unsigned long long n = o.count;
-  if (idx >= n) return;
+  if (idx >= n) return true;
+  const float r_near = R.r_turn >= 0.0f ? fminf(R.r_turn, R.r) : R.r;
+  const Verdict v = ray_verdict(s, src.state8, src.px, src.py, r_near, R.d_crit, refined);
+  const bool definite = event_class(R.term);
+  const bool critical = definite && (v.extra & GL_STATUS_NEAR_CRITICAL);
+  // the refine pass settles the branch as well as the class
+  if (definite && (v.extra & (GL_STATUS_NEAR_CRITICAL | GL_STATUS_BRANCH_UNSETTLED)) && !refined &&
+      (s.options & kOptRefine) && sink.rec) {
+    const unsigned int slot = atomicAdd(sink.count, 1u);
+    if (slot < sink.capacity) {
+      sink.rec[slot] = RefineRecord{frame_scene, idx};
+      return false;
+    }
+  }
+  uint32_t cls = R.term, extra = (definite ? v.extra : 0u) | (refined ? GL_STATUS_REFINED : 0u);
+  if (definite && (critical || (R.flags & (kFlagTied | kFlagTurningOrder | kFlagDrift)))) {
+    cls = kTermUncertain;
+    extra |= (R.term & 15u) << GL_STATUS_PROVISIONAL_SHIFT;
+  }
+  const uint32_t status = (cls & 15u) | ((R.candidates & 15u) << 4) | ((R.flags & 255u) << 8) | extra;
   if (o.steps) o.steps[idx] = R.accepted;
   if (o.rejected) o.rejected[idx] = R.rejected;
   if (o.travel_time) o.travel_time[idx] = R.dir * (R.t - R.ct);
@@ -351,8 +543,9 @@ __device__ __forceinline__ void write_ray(const RayT<float>& R, const OutPlanes&
   if (o.branch)
     o.branch[idx] = (min(R.turnings, 255u)) | (min(R.crossings, 255u) << 8) |
                     ((uint32_t(R.winding) & 255u) << 16) | ((R.polar_side & 3u) << 24);
-  write_terminal(o, idx, s, map, R.term, R.candidates, R.flags, R.dir, R.x, R.y, R.z, R.cx, R.cy,
-                 R.cz, R.px, R.py, R.pz, R.E);
+  write_terminal(o, idx, s, map, R.term, status, R.dir, R.x, R.y, R.z, R.cx, R.cy, R.cz, R.px,
+                 R.py, R.pz, R.E, R.winding, v.phi0, R.r_turn);
+  return true;
 }
 
 // Scene constants of the trace loop held in VGPRs: as wave-uniform values
@@ -360,25 +553,26 @@ __device__ __forceinline__ void write_ray(const RayT<float>& R, const OutPlanes&
 // scalar file then spills to VGPR lanes with a readlane per use.
 __device__ __forceinline__ void hold_in_vgprs(SceneT<float>& sc) {
   asm volatile("" : "+v"(sc.spin), "+v"(sc.spin2), "+v"(sc.charge2), "+v"(sc.chart));
-  asm volatile("" : "+v"(sc.r_capture), "+v"(sc.r_horizon), "+v"(sc.r_escape), "+v"(sc.disk_in),
+  asm volatile("" : "+v"(sc.r_capture), "+v"(sc.r_defer), "+v"(sc.r_escape), "+v"(sc.disk_in),
                "+v"(sc.disk_out));
   asm volatile("" : "+v"(sc.band), "+v"(sc.rtol), "+v"(sc.atol), "+v"(sc.h_min), "+v"(sc.h_max),
                "+v"(sc.h_max_per_r), "+v"(sc.tie_tol), "+v"(sc.drift_limit), "+v"(sc.guard));
   asm volatile("" : "+v"(sc.safety), "+v"(sc.min_factor), "+v"(sc.max_factor));
 }
 
-// Null re-projection on a schedule every lane of a wavefront shares: the
-// f64 branch runs for the whole wave at once instead of once per lane.
-__device__ __forceinline__ bool reproject_now(const SceneT<float>& sc, uint32_t wave_step) {
-  return sc.reproject_interval != 0u && (wave_step % sc.reproject_interval) == 0u;
+// Null re-projection every reproject_interval-th DP5 attempt of a ray
+// (attempt = 1, 2, ...: accepted and rejected attempts both count), so a
+// ray's arithmetic does not depend on which kernel or wavefront traces it.
+__device__ __forceinline__ bool reproject_now(const SceneT<float>& sc, uint32_t attempt) {
+  return sc.reproject_interval != 0u && (attempt % sc.reproject_interval) == 0u;
 }
 
-// Invariant drifts are sampled every fourth attempt (and at every
-// re-projection and termination), on the same wavefront-uniform clock.
-__device__ __forceinline__ bool drift_now(uint32_t wave_step) { return (wave_step & 3u) == 0u; }
+// Invariant drifts are sampled every fourth attempt of a ray (and at every
+// re-projection and termination).
+__device__ __forceinline__ bool drift_now(uint32_t attempt) { return (attempt & 3u) == 0u; }
 
 __device__ __forceinline__ bool wants_defer(const SceneT<float>& sc, const RayT<float>& R) {
-  return (sc.options & kOptF64Horizon) && sc.r_horizon > 0.0f && R.r < 1.05f * sc.r_horizon;
+  return (sc.options & kOptF64Horizon) && R.r < sc.r_defer;
 }
 
 __device__ __forceinline__ bool try_defer(const DeferBuf& df, const RayT<float>& R, uint32_t fs,

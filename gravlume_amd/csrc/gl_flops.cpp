@@ -124,7 +124,7 @@ uint64_t count_attempt() {
   sc.charge2 = 0.0;
   sc.chart = -1.0;
   sc.r_capture = 1.3411744 * (1 + 1e-6);
-  sc.r_horizon = 1.3411744;
+  sc.r_defer = 1.05 * 1.3411744;
   sc.r_escape = 1000.0;
   sc.disk_in = 2.0235931;
   sc.disk_out = 30.0;

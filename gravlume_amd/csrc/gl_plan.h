@@ -16,6 +16,21 @@ namespace gl {
 gl_status fail(gl_status st, const std::string& msg);
 }  // namespace gl
 
+namespace gl {
+// Device buffers of the f64 passes of a launch: the rays deferred to the
+// near-horizon finisher, the near-critical rays for the refine pass, and
+// their four counters (DeferBuf). The plan owns one for its eager calls;
+// every frame graph owns its own, so a later call that needs more room can
+// reallocate the plan's without touching memory a graph replays into.
+struct Workspace {
+  DeferRecord* defer = nullptr;
+  uint32_t defer_capacity = 0;
+  RefineRecord* refine = nullptr;
+  uint32_t refine_capacity = 0;
+  unsigned int* counters = nullptr;  // 64 bytes, the first four used
+};
+}  // namespace gl
+
 #define GL_HIP(call)                                                                       \
   do {                                                                                     \
     hipError_t e_ = (call);                                                                \
@@ -42,10 +57,8 @@ struct gl_trace_plan {
   std::vector<FrameScene> frame_shadow;  // what d_frame holds (or will hold)
   hipEvent_t frame_copied = nullptr;     // completion of the last staging copy
   bool frame_copy_pending = false;
-  DeferRecord* d_defer = nullptr;
-  unsigned int* d_defer_count = nullptr;
+  gl::Workspace ws;             // deferral and refine buffers of eager calls
   uint32_t* d_tiles = nullptr;  // per-scene tile orders, then tile costs
-  uint32_t defer_capacity = 0;
   int cu_count = 256;
   // whole-tile mode: the faster one on every workload measured
   gl_persistent_config pcfg = {256, 2, 8, 64, 2};
@@ -85,15 +98,30 @@ void plan_mark(gl_trace_plan* p, int which, hipStream_t st);
 // accelerated sky trace (gl_accel.hip).
 gl_status hip_fail(hipError_t e, const char* what);
 gl_status check_outputs(const gl_ray_outputs* o, size_t rays);
+OutPlanes to_planes(const gl_ray_outputs& o);
 FrameScene make_frame(uint32_t scene, uint32_t first, uint32_t rays, const gl_ray_outputs& o);
 gl_status upload_frame(gl_trace_plan* p, const std::vector<FrameScene>& table, hipStream_t st);
-gl_status ensure_defer(gl_trace_plan* p, size_t rays);
-gl_status launch_finisher(gl_trace_plan* p, hipStream_t st);
+// Grows `w` to hold the deferral records (f64) and refine records (refine)
+// of a launch of `rays` rays, each capped at 4M records.
+gl_status ensure_workspace(Workspace& w, size_t rays, bool f64, bool refine);
+void free_workspace(Workspace& w);
+// The kernels' view of `w`: no deferral unless f64, no refine pass unless refine.
+DeferBuf workspace_view(const Workspace& w, bool f64, bool refine);
+// Whether any scene of the plan (or scene `scene`) defers near-horizon
+// steps to f64, and whether it refines near-critical rays.
+bool any_f64_horizon(const gl_trace_plan* p);
+bool any_refine(const gl_trace_plan* p);
+bool scene_f64_horizon(const SceneDev& s);
+bool scene_refine(const SceneDev& s);
+// The near-horizon finisher (when df defers) and the refine pass (when df
+// refines) after a trace launch on `st`, reading the frame table `table`.
+gl_status launch_f64_passes(gl_trace_plan* p, const FrameScene* table, const DeferBuf& df,
+                            hipStream_t st);
 // The next whole-tile launch's tile order from this launch's tile costs, on
 // `st` after the launch.
 gl_status launch_tile_order(gl_trace_plan* p, const FrameScene* d_table, size_t n_table,
                             hipStream_t st);
 MapView map_view(const gl_trace_plan* p);
-DeferBuf defer_buf(const gl_trace_plan* p);
+
 __global__ void k_queue_reset(QueueDev* q, unsigned long long total, unsigned int* defer_count);
 }  // namespace gl

@@ -48,6 +48,21 @@ double dot(const Metric& m, const V4& a, const V4& b) {
   return s;
 }
 
+// The contraction and the sum of the magnitudes of its sixteen terms
+// (metric_dot_and_term_norm, spacetime.rs:463-483): the scale against which
+// the Gram check measures a residual.
+double dot_term_norm(const Metric& m, const V4& a, const V4& b, double* norm) {
+  double s = 0.0, n = 0.0;
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) {
+      const double t = m.g[i][j] * a.v[i] * b.v[j];
+      s += t;
+      n += std::fabs(t);
+    }
+  *norm = n;
+  return s;
+}
+
 // Gram-Schmidt against u (timelike, u.u = -1) and the given spacelike unit
 // vectors; false when the remainder is degenerate.
 bool orthonormalize(const Metric& m, const V4& u, const V4* basis, int nb, V4 seed, V4* out) {
@@ -193,7 +208,8 @@ int prepare_scene(const gl_scene_desc& d, SceneDev* out, SceneHost* host, std::s
                          d.minimum_step, d.maximum_step, d.maximum_step_per_radius,
                          d.event_tie_tolerance, d.arming_band, d.drift_limit,
                          d.subpixel[0], d.subpixel[1], d.step_safety, d.step_min_factor,
-                         d.step_max_factor};
+                         d.step_max_factor, d.near_critical_factor, d.branch_factor,
+                         d.refine_rtol, d.refine_atol};
   for (double v : vals)
     if (!std::isfinite(v)) return set_err(err, "scene value is not finite"), GL_ERR_INVALID_ARGUMENT;
   for (int i = 0; i < 4; ++i)
@@ -225,6 +241,22 @@ int prepare_scene(const gl_scene_desc& d, SceneDev* out, SceneHost* host, std::s
     const double mx = d.step_max_factor == 0.0 ? 5.0 : d.step_max_factor;
     if (!(sf > 0.0 && sf <= 1.0) || !(mn > 0.0 && mn <= 1.0) || !(mx >= 1.0 && mx <= 1.0e3))
       return set_err(err, "step controller factors must satisfy 0 < safety <= 1, 0 < min <= 1 <= max"),
+             GL_ERR_INVALID_ARGUMENT;
+  }
+  if (d.refine_rtol < 0.0 || d.refine_atol < 0.0 ||
+      (d.refine_rtol > 0.0 && !(d.refine_rtol >= 1e-15 && d.refine_rtol <= 1e-3)) ||
+      (d.refine_atol > 0.0 && !(d.refine_atol >= 1e-18 && d.refine_atol <= 1e-3)))
+    return set_err(err, "refine tolerances must lie in [1e-15, 1e-3] (rtol) and [1e-18, 1e-3] "
+                        "(atol), or be 0 for the defaults"),
+           GL_ERR_INVALID_ARGUMENT;
+  if (d.near_critical_factor > 1.0e6 || d.branch_factor > 1.0e9)
+    return set_err(err, "near-critical factor must not exceed 1e6, the branch factor 1e9"),
+           GL_ERR_INVALID_ARGUMENT;
+  {
+    const double nf = d.near_critical_factor == 0.0 ? 16.0 : d.near_critical_factor;
+    const double bf = d.branch_factor == 0.0 ? 4096.0 : d.branch_factor;
+    if (nf > 0.0 && !(bf >= nf))
+      return set_err(err, "the branch factor must be at least the near-critical factor"),
              GL_ERR_INVALID_ARGUMENT;
   }
   if (d.max_accepted_steps == 0 || d.max_consecutive_rejects == 0)
@@ -275,7 +307,8 @@ int prepare_scene(const gl_scene_desc& d, SceneDev* out, SceneHost* host, std::s
   V4 up;
   V4 up_seed = {{0.0, d.up_hint[0], d.up_hint[1], d.up_hint[2]}};
   if (!orthonormalize(m, u, &sight, 1, up_seed, &up)) {
-    // Fallback: the coordinate axis with the largest orthogonal remainder.
+    // Fallback: the coordinate axis whose normalized remainder has the
+    // largest spatial component (best_coordinate_axis, observer.rs:319-352).
     double best = -1.0;
     bool found = false;
     for (int a = 1; a <= 3; ++a) {
@@ -284,7 +317,7 @@ int prepare_scene(const gl_scene_desc& d, SceneDev* out, SceneHost* host, std::s
       V4 c;
       if (!orthonormalize(m, u, &sight, 1, s, &c)) continue;
       double score = std::fmax(std::fabs(c.v[1]), std::fmax(std::fabs(c.v[2]), std::fabs(c.v[3])));
-      if (score >= best) {
+      if (score >= best * (1.0 - 1.0e-9)) {  // ties go to the later axis
         best = score;
         up = c;
         found = true;
@@ -293,7 +326,12 @@ int prepare_scene(const gl_scene_desc& d, SceneDev* out, SceneHost* host, std::s
     if (!found) return set_err(err, "no stable image-up axis"), GL_ERR_DOMAIN;
   }
   // Right leg: the remaining unit direction, oriented so that
-  // det(u, right, up, arrival) > 0.
+  // det(u, right, up, arrival) > 0. Axes are compared by the largest
+  // spatial component of their normalized remainder, the later axis winning
+  // ties (Iterator::max_by). Two axes that normalize to the same direction
+  // tie exactly in exact arithmetic; the one with the tiny remainder carries
+  // its cancellation error into its score and, picked by that last-bit
+  // accident, fails the Gram budget, so scores within a relative 1e-9 tie.
   V4 right;
   {
     V4 basis[2] = {sight, up};
@@ -305,7 +343,7 @@ int prepare_scene(const gl_scene_desc& d, SceneDev* out, SceneHost* host, std::s
       V4 c;
       if (!orthonormalize(m, u, basis, 2, s, &c)) continue;
       double score = std::fmax(std::fabs(c.v[1]), std::fmax(std::fabs(c.v[2]), std::fabs(c.v[3])));
-      if (score >= best) {
+      if (score >= best * (1.0 - 1.0e-9)) {  // ties go to the later axis
         best = score;
         right = c;
         found = true;
@@ -316,14 +354,19 @@ int prepare_scene(const gl_scene_desc& d, SceneDev* out, SceneHost* host, std::s
     if (det4(cols) < 0.0)
       for (int i = 0; i < 4; ++i) right.v[i] = -right.v[i];
   }
-  // Gram check of the tetrad (observer.rs:354-376, 1e-12 budget).
+  // Gram check of the tetrad (observer.rs:354-376, 1e-12 budget): each
+  // residual relative to the magnitude of the terms it cancels, so that a
+  // frame in a strong-field metric with large components is judged by its
+  // rounding, not by the size of g.
   {
     V4 b[4] = {u, right, up, arrival};
     double worst = 0.0;
     for (int i = 0; i < 4; ++i)
       for (int j = 0; j < 4; ++j) {
         double e = i == j ? (i == 0 ? -1.0 : 1.0) : 0.0;
-        worst = std::fmax(worst, std::fabs(dot(m, b[i], b[j]) - e));
+        double norm = 0.0;
+        const double v = dot_term_norm(m, b[i], b[j], &norm);
+        worst = std::fmax(worst, std::fabs(v - e) / std::fmax(norm, 1.0));
       }
     if (!(worst <= 1e-12)) return set_err(err, "observer frame failed its Gram check"), GL_ERR_DOMAIN;
   }
@@ -398,6 +441,24 @@ int prepare_scene(const gl_scene_desc& d, SceneDev* out, SceneHost* host, std::s
   s.slab_emission_d = slab ? d.slab_emission : 0.0;
   s.slab_temperature_d = slab ? d.slab_temperature : 0.0;
   s.lut = nullptr;  // set by the plan when it uploads the band table
+  for (int i = 0; i < 3; ++i) s.obs_d[i] = ox[i];
+  s.u_t_d = u.v[0];
+  for (int i = 0; i < 4; ++i) {
+    s.right_d[i] = right.v[i];
+    s.up_d[i] = up.v[i];
+    s.arrival_d[i] = arrival.v[i];
+  }
+  s.tan_half_fov_d = thf;
+  s.aspect_d = (double)d.width / (double)d.height;
+  s.sub_x_d = d.subpixel[0];
+  s.sub_y_d = d.subpixel[1];
+  s.r_observer_d = r_obs;
+  s.near_critical_factor = d.near_critical_factor == 0.0 ? 16.0f
+                           : (d.near_critical_factor < 0.0 ? -1.0f : (float)d.near_critical_factor);
+  s.branch_factor = s.near_critical_factor < 0.0f ? -1.0f
+                     : (d.branch_factor == 0.0 ? 4096.0f : (float)d.branch_factor);
+  s.refine_rtol = d.refine_rtol == 0.0 ? 1e-12f : (float)d.refine_rtol;
+  s.refine_atol = d.refine_atol == 0.0 ? 1e-14f : (float)d.refine_atol;
   if (!std::isfinite(s.tan_half_fov) || !std::isfinite(s.u_t))
     return set_err(err, "scene not representable in f32"), GL_ERR_NOT_REPRESENTABLE;
 

@@ -37,6 +37,7 @@ __global__ void __launch_bounds__(64) k_plain_image(const SceneDev* __restrict__
   const FrameScene& fsc = frame[0];
   const SceneDev& s = scenes[fsc.scene];
   SceneT<float> sc = scene_f32(s);
+  if (fsc.out.turning_radius) sc.options |= kOptTurningRadius;
   hold_in_vgprs(sc);
   uint32_t tx = tile_x0 + blockIdx.x % tiles_x;
   uint32_t ty = tile_y0 + blockIdx.x / tiles_x;
@@ -50,7 +51,7 @@ __global__ void __launch_bounds__(64) k_plain_image(const SceneDev* __restrict__
     if (wants_defer(sc, R) && try_defer(df, R, 0u, idx)) return;
     step<float>(sc, R, s.omega, reproject_now(sc, it), drift_now(it));
   }
-  write_ray(R, fsc.out, idx, s, map);
+  write_ray(R, fsc.out, idx, s, map, pixel_source(px, py), refine_sink(df), 0u);
 }
 
 __global__ void __launch_bounds__(64) k_geodesic_batch(const SceneDev* __restrict__ scenes,
@@ -60,7 +61,8 @@ __global__ void __launch_bounds__(64) k_geodesic_batch(const SceneDev* __restric
                                                        uint32_t n, MapView map, DeferBuf df) {
   const FrameScene& fsc = frame[0];
   const SceneDev& s = scenes[fsc.scene];
-  const SceneT<float> sc = scene_f32(s);
+  SceneT<float> sc = scene_f32(s);
+  if (fsc.out.turning_radius) sc.options |= kOptTurningRadius;
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   RayT<float> R;
@@ -69,7 +71,7 @@ __global__ void __launch_bounds__(64) k_geodesic_batch(const SceneDev* __restric
     if (wants_defer(sc, R) && try_defer(df, R, 0u, i)) return;
     step<float>(sc, R, s.omega, reproject_now(sc, it), drift_now(it));
   }
-  write_ray(R, fsc.out, i, s, map);
+  write_ray(R, fsc.out, i, s, map, batch_source(states + 8 * size_t(i)), refine_sink(df), 0u);
 }
 
 __device__ __forceinline__ uint32_t find_frame_scene(const FrameScene* frame, uint32_t n,
@@ -100,12 +102,13 @@ __global__ void __launch_bounds__(256, kWaves) k_persistent(const SceneDev* __re
   float omega = 0.0f;
   if (kSingle) {
     sc = scene_f32(scenes[frame[0].scene]);
+    if (frame[0].out.turning_radius) sc.options |= kOptTurningRadius;
     omega = scenes[frame[0].scene].omega;
     hold_in_vgprs(sc);
   }
   uint32_t lane_attempts = 0;  // per lane: DP5 attempts made
   uint32_t refills = 0;
-  uint32_t wave_steps = 0;  // wave-uniform attempt counter (lane slots / 64)
+  uint32_t wave_steps = 0;  // wave attempt slots (lane slots / 64)
   bool drained = false;  // wave-uniform: the queue has no items left
   for (;;) {
     if (!drained) {
@@ -127,6 +130,7 @@ __global__ void __launch_bounds__(256, kWaves) k_persistent(const SceneDev* __re
             const SceneDev& s = scenes[f.scene];
             if (!kSingle) {
               sc = scene_f32(s);
+              if (f.out.turning_radius) sc.options |= kOptTurningRadius;
               omega = s.omega;
             }
             uint32_t local = uint32_t(item - f.first_item);
@@ -150,7 +154,7 @@ __global__ void __launch_bounds__(256, kWaves) k_persistent(const SceneDev* __re
             if (have) {
               active = true;
               if (R.term != kTermActive) {
-                write_ray(R, f.out, idx, s, map);
+                write_ray(R, f.out, idx, s, map, frame_source(f, s, idx), refine_sink(df), fs);
                 active = false;
               }
             }
@@ -169,10 +173,15 @@ __global__ void __launch_bounds__(256, kWaves) k_persistent(const SceneDev* __re
         if (wants_defer(sc, R) && try_defer(df, R, fs, idx)) {
           active = false;
         } else {
-          step<float>(sc, R, omega, reproject_now(sc, wave_steps), drift_now(wave_steps));
+          // re-projection and drift sampling on the ray's own attempt clock,
+          // as in the plain and whole-tile kernels: a ray's arithmetic does
+          // not depend on the wavefront it was handed to
+          const uint32_t attempt = R.accepted + R.rejected + 1u;
+          step<float>(sc, R, omega, reproject_now(sc, attempt), drift_now(attempt));
           if (R.term != kTermActive) {
             const FrameScene& f = frame[fs];
-            write_ray(R, f.out, idx, scenes[f.scene], map);
+            const SceneDev& s = scenes[f.scene];
+            write_ray(R, f.out, idx, s, map, frame_source(f, s, idx), refine_sink(df), fs);
             active = false;
           }
         }
@@ -213,6 +222,7 @@ __global__ void __launch_bounds__(256, kWaves) k_persistent_tiles(
     const FrameScene& f = frame[fs];
     const SceneDev& s = scenes[f.scene];
     SceneT<float> sc = scene_f32(s);
+    if (f.out.turning_radius) sc.options |= kOptTurningRadius;
     hold_in_vgprs(sc);
     const uint32_t local = uint32_t(item - f.first_item);
     uint32_t it = 0, idx = 0, tile = 0xffffffffu;
@@ -246,7 +256,7 @@ __global__ void __launch_bounds__(256, kWaves) k_persistent_tiles(
         step<float>(sc, R, s.omega, reproject_now(sc, it), drift_now(it));
       }
       lane_attempts += it - 1;
-      if (!deferred) write_ray(R, f.out, idx, s, map);
+      if (!deferred) write_ray(R, f.out, idx, s, map, frame_source(f, s, idx), refine_sink(df), fs);
     }
     uint32_t longest = it;  // wave iterations of this tile: its longest lane
     for (int off = 32; off > 0; off >>= 1) longest = max(longest, (uint32_t)__shfl_xor(longest, off));
@@ -358,6 +368,44 @@ __global__ void __launch_bounds__(kOrderThreads) k_tile_order(const SceneDev* __
   }
 }
 
+// The f32 view of an f64 ray for the output writer: the position rounded
+// with its f64 remainder as the compensation term, so the terminal
+// observables are still formed from the f64 position.
+__device__ __forceinline__ RayT<float> narrow_ray(const RayT<double>& R) {
+  RayT<float> W;
+  W.x = float(R.x);
+  W.y = float(R.y);
+  W.z = float(R.z);
+  W.cx = float(double(W.x) - (R.x - R.cx));
+  W.cy = float(double(W.y) - (R.y - R.cy));
+  W.cz = float(double(W.z) - (R.z - R.cz));
+  W.px = float(R.px);
+  W.py = float(R.py);
+  W.pz = float(R.pz);
+  W.E = float(R.E);
+  W.term = R.term;
+  W.candidates = R.candidates;
+  W.flags = R.flags;
+  W.accepted = R.accepted;
+  W.rejected = R.rejected;
+  W.t = float(R.t - R.ct);
+  W.ct = 0.0f;
+  W.lam = float(R.lam - R.clam);
+  W.clam = 0.0f;
+  W.dir = R.dir;
+  W.d_null = float(R.d_null);
+  W.d_lz = float(R.d_lz);
+  W.d_q = float(R.d_q);
+  W.turnings = R.turnings;
+  W.crossings = R.crossings;
+  W.winding = R.winding;
+  W.polar_side = R.polar_side;
+  W.r_turn = float(R.r_turn);
+  W.r_m = float(R.r_m);
+  W.d_crit = float(R.d_crit);
+  return W;
+}
+
 __global__ void __launch_bounds__(64) k_finish_f64(const SceneDev* __restrict__ scenes,
                                                    const FrameScene* __restrict__ frame,
                                                    MapView map, DeferBuf df) {
@@ -372,7 +420,8 @@ __global__ void __launch_bounds__(64) k_finish_f64(const SceneDev* __restrict__ 
     const DeferRecord& rec = df.rec[i];
     const FrameScene& f = frame[rec.frame_scene];
     const SceneDev& s = scenes[f.scene];
-    const SceneT<double> sd = scene_f64(s);
+    SceneT<double> sd = scene_f64(s);
+    if (f.out.turning_radius) sd.options |= kOptTurningRadius;
     const RayT<float>& Rf = rec.ray;
     RayT<double> R;
     R.x = double(Rf.x) - double(Rf.cx);
@@ -405,41 +454,52 @@ __global__ void __launch_bounds__(64) k_finish_f64(const SceneDev* __restrict__ 
     R.flags = Rf.flags;
     R.polar_side = Rf.polar_side;
     R.dir = Rf.dir;
+    R.r_turn = Rf.r_turn;
+    R.r_m = Rf.r_m;
+    R.d_crit = Rf.d_crit;
     if (!refresh_derivative<double>(sd, R)) {
       R.term = kTermFailure;
       R.flags |= kFlagNonFinite;
     }
     for (uint32_t it = 1; R.term == kTermActive; ++it)
       step<double>(sd, R, double(s.omega), false, drift_now(it));
-    RayT<float> W;
-    W.x = float(R.x);
-    W.y = float(R.y);
-    W.z = float(R.z);
-    W.cx = float(double(W.x) - (R.x - R.cx));
-    W.cy = float(double(W.y) - (R.y - R.cy));
-    W.cz = float(double(W.z) - (R.z - R.cz));
-    W.px = float(R.px);
-    W.py = float(R.py);
-    W.pz = float(R.pz);
-    W.E = float(R.E);
-    W.term = R.term;
-    W.candidates = R.candidates;
-    W.flags = R.flags;
-    W.accepted = R.accepted;
-    W.rejected = R.rejected;
-    W.t = float(R.t);
-    W.ct = 0.0f;
-    W.lam = float(R.lam);
-    W.clam = 0.0f;
-    W.dir = R.dir;
-    W.d_null = float(R.d_null);
-    W.d_lz = float(R.d_lz);
-    W.d_q = float(R.d_q);
-    W.turnings = R.turnings;
-    W.crossings = R.crossings;
-    W.winding = R.winding;
-    W.polar_side = R.polar_side;
-    write_ray(W, f.out, rec.index, s, map);
+    write_ray(narrow_ray(R), f.out, rec.index, s, map, frame_source(f, s, rec.index),
+              refine_sink(df), rec.frame_scene);
+  }
+}
+
+// The f64 refine pass (GL_OPT_NEAR_CRITICAL_REFINE): every near-critical
+// ray of the launch is traced again from its exact f64 initial state, in
+// f64 throughout at the scene's refine tolerances, and written with the
+// near-critical test of that trace (an f64 start and f64 drifts). One ray
+// per lane from a claim cursor (df.count[3]), like the finisher.
+__global__ void __launch_bounds__(64) k_refine_f64(const SceneDev* __restrict__ scenes,
+                                                   const FrameScene* __restrict__ frame,
+                                                   MapView map, DeferBuf df) {
+  unsigned int n = df.count[2];
+  if (n > df.refine_capacity) n = df.refine_capacity;
+  for (;;) {
+    const unsigned int i = atomicAdd(df.count + 3, 1u);
+    if (i >= n) break;
+    const RefineRecord rec = df.refine[i];
+    const FrameScene& f = frame[rec.frame_scene];
+    const SceneDev& s = scenes[f.scene];
+    SceneT<double> sd = scene_f64(s);
+    sd.rtol = double(s.refine_rtol);
+    sd.atol = double(s.refine_atol);
+    if (f.out.turning_radius) sd.options |= kOptTurningRadius;
+    const RaySource src = frame_source(f, s, rec.index);
+    double y[8];
+    source_state(s, src.state8, src.px, src.py, y);
+    const float dir = f.states ? (f.dirs[rec.index] >= 0.0f ? 1.0f : -1.0f) : -1.0f;
+    const uint32_t side = f.states ? (y[3] > 0.0 ? 2u : (y[3] < 0.0 ? 0u : 1u)) : s.polar_side;
+    RayT<double> R;
+    init_ray<double>(sd, R, y[1], y[2], y[3], 0.0, 0.0, 0.0, y[5], y[6], y[7], -y[4], dir, side);
+    R.h = double(dir) * double(s.h_init);
+    R.r_m = critical_radius(s, y);
+    for (uint32_t it = 1; R.term == kTermActive; ++it)
+      step<double>(sd, R, double(s.omega), false, drift_now(it));
+    write_ray(narrow_ray(R), f.out, rec.index, s, map, src, no_refine(), rec.frame_scene, true);
   }
 }
 
@@ -454,6 +514,8 @@ __global__ void k_queue_reset(QueueDev* q, unsigned long long total, unsigned in
     if (defer_count) {
       defer_count[0] = 0;  // deferred rays
       defer_count[1] = 0;  // finisher claim cursor
+      defer_count[2] = 0;  // refine records
+      defer_count[3] = 0;  // refine claim cursor
     }
   }
 }
@@ -491,6 +553,9 @@ OutPlanes to_planes(const gl_ray_outputs& o) {
   p.branch = o.branch;
   p.radiance = o.radiance;
   p.rgba16f = o.rgba16f;
+  p.event_residual = o.event_residual;
+  p.turning_radius = o.turning_radius;
+  p.azimuth_advance = o.azimuth_advance;
   p.count = o.count;
   return p;
 }
@@ -501,6 +566,7 @@ struct gl_frame_graph {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   FrameScene* d_table = nullptr;
+  gl::Workspace ws;  // the graph's own deferral / refine buffers and counters
 };
 
 namespace gl {
@@ -556,16 +622,47 @@ FrameScene make_frame(uint32_t scene, uint32_t first, uint32_t rays, const gl_ra
   return f;
 }
 
-gl_status ensure_defer(gl_trace_plan* p, size_t rays) {
-  // Near-horizon deferral holds at most one record per ray, capped at 4M rays
-  // (~1 GB); beyond the cap a ray keeps stepping in f32.
-  size_t want = std::min<size_t>(rays, size_t(1) << 22);
-  if (want <= p->defer_capacity && p->d_defer) return GL_OK;
-  if (p->d_defer) (void)hipFree(p->d_defer);
-  p->d_defer = nullptr;
-  GL_HIP(hipMalloc(&p->d_defer, want * sizeof(DeferRecord)));
-  p->defer_capacity = uint32_t(want);
+gl_status ensure_workspace(Workspace& w, size_t rays, bool f64, bool refine) {
+  // At most one record per ray, capped at 4M rays (~0.8 GB of deferral
+  // records); beyond the cap a ray keeps stepping in f32 (deferral) or is
+  // written as near-critical and Uncertain (refine).
+  const size_t want = std::min<size_t>(std::max<size_t>(rays, 1), size_t(1) << 22);
+  if (!w.counters) {
+    GL_HIP(hipMalloc(&w.counters, 64));
+    GL_HIP(hipMemset(w.counters, 0, 64));
+  }
+  if (f64 && (want > w.defer_capacity || !w.defer)) {
+    if (w.defer) (void)hipFree(w.defer);
+    w.defer = nullptr;
+    w.defer_capacity = 0;
+    GL_HIP(hipMalloc(&w.defer, want * sizeof(DeferRecord)));
+    w.defer_capacity = uint32_t(want);
+  }
+  if (refine && (want > w.refine_capacity || !w.refine)) {
+    if (w.refine) (void)hipFree(w.refine);
+    w.refine = nullptr;
+    w.refine_capacity = 0;
+    GL_HIP(hipMalloc(&w.refine, want * sizeof(RefineRecord)));
+    w.refine_capacity = uint32_t(want);
+  }
   return GL_OK;
+}
+
+void free_workspace(Workspace& w) {
+  if (w.defer) (void)hipFree(w.defer);
+  if (w.refine) (void)hipFree(w.refine);
+  if (w.counters) (void)hipFree(w.counters);
+  w = Workspace();
+}
+
+DeferBuf workspace_view(const Workspace& w, bool f64, bool refine) {
+  DeferBuf d;
+  d.rec = f64 ? w.defer : nullptr;
+  d.count = w.counters;
+  d.capacity = f64 ? w.defer_capacity : 0u;
+  d.refine = refine ? w.refine : nullptr;
+  d.refine_capacity = refine ? w.refine_capacity : 0u;
+  return d;
 }
 
 gl_status harvest(gl_trace_plan* p, gl_trace_plan::TimedLaunch& t) {
@@ -623,17 +720,20 @@ MapView map_view(const gl_trace_plan* p) {
   return m;
 }
 
-DeferBuf defer_buf(const gl_trace_plan* p) {
-  DeferBuf d;
-  d.rec = p->d_defer;
-  d.count = p->d_defer_count;
-  d.capacity = p->defer_capacity;
-  return d;
+bool scene_f64_horizon(const SceneDev& s) { return (s.options & kOptF64Horizon) != 0; }
+bool scene_refine(const SceneDev& s) {
+  return (s.options & kOptRefine) != 0 && s.near_critical_factor > 0.0f && s.r_horizon > 0.0f;
 }
 
 bool any_f64_horizon(const gl_trace_plan* p) {
   for (const auto& s : p->scenes)
-    if (s.options & kOptF64Horizon) return true;
+    if (scene_f64_horizon(s)) return true;
+  return false;
+}
+
+bool any_refine(const gl_trace_plan* p) {
+  for (const auto& s : p->scenes)
+    if (scene_refine(s)) return true;
   return false;
 }
 
@@ -655,13 +755,22 @@ const std::vector<float>* blackbody_table() {
   return table.empty() ? nullptr : &table;
 }
 
-// Launch the finisher over however many rays were deferred (read on device).
-gl_status launch_finisher(gl_trace_plan* p, hipStream_t st) {
-  if (!p->d_defer) return GL_OK;
+// The finisher over however many rays were deferred, then the refine pass
+// over however many were near-critical (both counts read on the device).
+// Timing events 2 and 3 bracket both.
+gl_status launch_f64_passes(gl_trace_plan* p, const FrameScene* table, const DeferBuf& df,
+                            hipStream_t st) {
+  if (!df.rec && !df.refine) return GL_OK;
   mark(p, 2, st);
-  hipLaunchKernelGGL(k_finish_f64, dim3(std::max(1, p->cu_count * 4)), dim3(64), 0, st,
-                     p->d_scenes, p->d_frame, map_view(p), defer_buf(p));
-  GL_HIP(hipGetLastError());
+  const dim3 grid(std::max(1, p->cu_count * 4));
+  if (df.rec) {
+    hipLaunchKernelGGL(k_finish_f64, grid, dim3(64), 0, st, p->d_scenes, table, map_view(p), df);
+    GL_HIP(hipGetLastError());
+  }
+  if (df.refine) {
+    hipLaunchKernelGGL(k_refine_f64, grid, dim3(64), 0, st, p->d_scenes, table, map_view(p), df);
+    GL_HIP(hipGetLastError());
+  }
   mark(p, 3, st);
   return GL_OK;
 }
@@ -771,18 +880,16 @@ const void* persistent_kernel(bool single, uint32_t waves, bool tiles) {
 }
 
 // Queue reset, persistent kernel and (when any scene asks for f64 horizon
-// steps) the f64 finisher, all on `st`, reading the frame table d_table.
+// steps or the near-critical refine) the f64 passes, all on `st`, reading
+// the frame table d_table and using workspace `w` (the plan's, or a frame
+// graph's own).
 gl_status launch_persistent(gl_trace_plan* p, const FrameScene* d_table, size_t n_table,
                             unsigned long long total, size_t total_rays, hipStream_t st,
-                            const gl_persistent_config& c) {
-  gl_status s = GL_OK;
-  bool f64 = any_f64_horizon(p);
-  if (f64) {
-    s = ensure_defer(p, total_rays);
-    if (s) return s;
-  }
-  hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, st, p->d_queue, total,
-                     p->d_defer_count);
+                            const gl_persistent_config& c, Workspace& w) {
+  const bool f64 = any_f64_horizon(p), refine = any_refine(p);
+  gl_status s = ensure_workspace(w, total_rays, f64, refine);
+  if (s) return s;
+  hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, st, p->d_queue, total, w.counters);
   GL_HIP(hipGetLastError());
   // Every workgroup must be resident at once: clamp the requested blocks
   // per CU to what the kernel's register budget allows.
@@ -797,8 +904,7 @@ gl_status launch_persistent(gl_trace_plan* p, const FrameScene* d_table, size_t 
   int per_cu = std::max(1, std::min(int(c.blocks_per_cu), resident));
   dim3 grid(std::max(1, p->cu_count * per_cu));
   dim3 block(c.block_size);
-  DeferBuf df = defer_buf(p);
-  if (!f64) df.rec = nullptr;
+  const DeferBuf df = workspace_view(w, f64, refine);
   mark(p, 0, st);
   {
     const SceneDev* a0 = p->d_scenes;
@@ -813,13 +919,8 @@ gl_status launch_persistent(gl_trace_plan* p, const FrameScene* d_table, size_t 
   }
   GL_HIP(hipGetLastError());
   mark(p, 1, st);
-  if (df.rec) {
-    mark(p, 2, st);
-    hipLaunchKernelGGL(k_finish_f64, dim3(std::max(1, p->cu_count * 4)), dim3(64), 0, st,
-                       p->d_scenes, d_table, map_view(p), df);
-    GL_HIP(hipGetLastError());
-    mark(p, 3, st);
-  }
+  s = launch_f64_passes(p, d_table, df, st);
+  if (s) return s;
   // the next whole-tile launch hands out this launch's longest tiles first
   if (tiles) return launch_tile_order(p, d_table, n_table, st);
   return GL_OK;
@@ -856,7 +957,7 @@ gl_status persistent_frame(gl_trace_plan* p, const uint32_t* scenes, size_t n_sc
   if (s) return s;
   s = upload_frame(p, table, st);
   if (s) return s;
-  return launch_persistent(p, p->d_frame, table.size(), total, rays, st, p->pcfg);
+  return launch_persistent(p, p->d_frame, table.size(), total, rays, st, p->pcfg, p->ws);
 }
 
 }  // namespace gl
@@ -952,8 +1053,8 @@ gl_status gl_trace_plan_create(const gl_scene_desc* scenes, size_t n_scenes,
     return cleanup(e, "upload scenes");
   if ((e = hipMalloc(&p->d_queue, sizeof(QueueDev))) != hipSuccess) return cleanup(e, "hipMalloc queue");
   if ((e = hipMemset(p->d_queue, 0, sizeof(QueueDev))) != hipSuccess) return cleanup(e, "clear queue");
-  if ((e = hipMalloc(&p->d_defer_count, 64)) != hipSuccess) return cleanup(e, "hipMalloc defer count");
-  if ((e = hipMemset(p->d_defer_count, 0, 64)) != hipSuccess) return cleanup(e, "clear defer count");
+  if ((e = hipMalloc(&p->ws.counters, 64)) != hipSuccess) return cleanup(e, "hipMalloc counters");
+  if ((e = hipMemset(p->ws.counters, 0, 64)) != hipSuccess) return cleanup(e, "clear counters");
   if (escape_map_rgb) {
     if (map_width == 0 || map_height == 0) {
       gl_trace_plan_destroy(p);
@@ -981,8 +1082,7 @@ gl_status gl_trace_plan_destroy(gl_trace_plan* p) {
   if (p->d_queue) (void)hipFree(p->d_queue);
   if (p->d_frame) (void)hipFree(p->d_frame);
   if (p->h_frame) (void)hipHostFree(p->h_frame);
-  if (p->d_defer) (void)hipFree(p->d_defer);
-  if (p->d_defer_count) (void)hipFree(p->d_defer_count);
+  free_workspace(p->ws);
   if (p->d_tiles) (void)hipFree(p->d_tiles);
   if (p->frame_copied) (void)hipEventDestroy(p->frame_copied);
   for (auto& t : p->timed)
@@ -1012,24 +1112,20 @@ gl_status gl_trace_encode(gl_trace_plan* p, const gl_tile_region* region,
   gl_status s = check_outputs(out, size_t(sd.width) * sd.height);
   if (s) return s;
   hipStream_t st = (hipStream_t)stream;
-  bool f64 = (sd.options & kOptF64Horizon) != 0;
-  if (f64) {
-    s = ensure_defer(p, size_t(region->tiles_x) * region->tiles_y * 64);
-    if (s) return s;
-  }
+  const bool f64 = scene_f64_horizon(sd), refine = scene_refine(sd);
+  s = ensure_workspace(p->ws, size_t(region->tiles_x) * region->tiles_y * 64, f64, refine);
+  if (s) return s;
   s = upload_frame(p, {make_frame(region->scene, 0, sd.width * sd.height, *out)}, st);
   if (s) return s;
-  hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, st, p->d_queue, 0ull, p->d_defer_count);
-  DeferBuf df = defer_buf(p);
-  if (!f64) df.rec = nullptr;
+  hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, st, p->d_queue, 0ull, p->ws.counters);
+  const DeferBuf df = workspace_view(p->ws, f64, refine);
   size_t blocks = size_t(region->tiles_x) * region->tiles_y;
   mark(p, 0, st);
   hipLaunchKernelGGL(k_plain_image, dim3(uint32_t(blocks)), dim3(64), 0, st, p->d_scenes,
                      p->d_frame, region->tile_x, region->tile_y, region->tiles_x, map_view(p), df);
   GL_HIP(hipGetLastError());
   mark(p, 1, st);
-  if (f64) return launch_finisher(p, st);
-  return GL_OK;
+  return launch_f64_passes(p, p->d_frame, df, st);
 }
 
 gl_status gl_trace_encode_persistent(gl_trace_plan* p, const uint32_t* scenes, size_t n_scenes,
@@ -1063,7 +1159,7 @@ gl_status gl_trace_encode_work(gl_trace_plan* p, const gl_work_item* items, size
   hipStream_t st = (hipStream_t)stream;
   s = upload_frame(p, table, st);
   if (s) return s;
-  return launch_persistent(p, p->d_frame, table.size(), total, rays, st, p->pcfg);
+  return launch_persistent(p, p->d_frame, table.size(), total, rays, st, p->pcfg, p->ws);
 }
 
 gl_status gl_geodesic_batch_trace_persistent(gl_trace_plan* p, uint32_t scene,
@@ -1088,7 +1184,7 @@ gl_status gl_geodesic_batch_trace_persistent(gl_trace_plan* p, uint32_t scene,
   f.dirs = directions;
   s = upload_frame(p, {f}, st);
   if (s) return s;
-  return launch_persistent(p, p->d_frame, 1, n, n, st, c);
+  return launch_persistent(p, p->d_frame, 1, n, n, st, c, p->ws);
 }
 
 
@@ -1103,23 +1199,22 @@ gl_status gl_geodesic_batch_trace(gl_trace_plan* p, uint32_t scene, const double
   gl_status s = check_outputs(out, n);
   if (s) return s;
   hipStream_t st = (hipStream_t)stream;
-  bool f64 = (p->scenes[scene].options & kOptF64Horizon) != 0;
-  if (f64) {
-    s = ensure_defer(p, n);
-    if (s) return s;
-  }
-  s = upload_frame(p, {make_frame(scene, 0, uint32_t(n), *out)}, st);
+  const bool f64 = scene_f64_horizon(p->scenes[scene]), refine = scene_refine(p->scenes[scene]);
+  s = ensure_workspace(p->ws, n, f64, refine);
   if (s) return s;
-  hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, st, p->d_queue, 0ull, p->d_defer_count);
-  DeferBuf df = defer_buf(p);
-  if (!f64) df.rec = nullptr;
+  FrameScene f = make_frame(scene, 0, uint32_t(n), *out);
+  f.states = states8;  // the refine pass rebuilds a request's start from its state
+  f.dirs = directions;
+  s = upload_frame(p, {f}, st);
+  if (s) return s;
+  hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, st, p->d_queue, 0ull, p->ws.counters);
+  const DeferBuf df = workspace_view(p->ws, f64, refine);
   mark(p, 0, st);
   hipLaunchKernelGGL(k_geodesic_batch, dim3(uint32_t((n + 63) / 64)), dim3(64), 0, st,
                      p->d_scenes, p->d_frame, states8, directions, uint32_t(n), map_view(p), df);
   GL_HIP(hipGetLastError());
   mark(p, 1, st);
-  if (f64) return launch_finisher(p, st);
-  return GL_OK;
+  return launch_f64_passes(p, p->d_frame, df, st);
 }
 
 gl_status gl_trace_plan_queue_stats(gl_trace_plan* p, gl_queue_stats* stats, void* stream) {
@@ -1177,11 +1272,14 @@ gl_status gl_frame_graph_create(gl_trace_plan* p, const uint32_t* scenes, size_t
   size_t rays = 0;
   gl_status s = build_frame_table(p, scenes, n_scenes, outputs, &table, &total, &rays);
   if (s) return s;
-  if (any_f64_horizon(p)) {
-    s = ensure_defer(p, rays);  // allocate before capture
-    if (s) return s;
-  }
   auto* g = new gl_frame_graph();
+  // The graph's own workspace, allocated before capture: eager calls on the
+  // plan may later grow or replace the plan's without affecting replays.
+  s = ensure_workspace(g->ws, rays, any_f64_horizon(p), any_refine(p));
+  if (s) {
+    gl_frame_graph_destroy(g);
+    return s;
+  }
   // The graph owns its frame table, written once before capture.
   hipError_t e = hipMalloc(&g->d_table, table.size() * sizeof(FrameScene));
   if (e == hipSuccess)
@@ -1196,7 +1294,7 @@ gl_status gl_frame_graph_create(gl_trace_plan* p, const uint32_t* scenes, size_t
     return hip_fail(e, "hipStreamBeginCapture");
   }
   p->capturing = true;
-  s = launch_persistent(p, g->d_table, table.size(), total, rays, st, p->pcfg);
+  s = launch_persistent(p, g->d_table, table.size(), total, rays, st, p->pcfg, g->ws);
   p->capturing = false;
   hipGraph_t graph = nullptr;
   e = hipStreamEndCapture(st, &graph);
@@ -1230,6 +1328,7 @@ gl_status gl_frame_graph_destroy(gl_frame_graph* g) {
   if (g->exec) (void)hipGraphExecDestroy(g->exec);
   if (g->graph) (void)hipGraphDestroy(g->graph);
   if (g->d_table) (void)hipFree(g->d_table);
+  free_workspace(g->ws);
   delete g;
   return GL_OK;
 }

@@ -7,9 +7,9 @@
 namespace gl {
 
 // One scene as the kernels read it: the f32 image of the validated f64 scene
-// plus the few f64 values the f64 islands and the surface radiance need.
-// 320 bytes, read once per ray into registers (the role of TraceUniforms,
-// trace/input.rs:19-35).
+// plus the f64 values the f64 islands, the f64 initial rays, the
+// near-critical test and the surface radiance need. 528 bytes, read through
+// the scalar cache (the role of TraceUniforms, trace/input.rs:19-35).
 struct alignas(16) SceneDev {
   // spacetime
   float mass, spin, spin2, charge2;
@@ -56,9 +56,20 @@ struct alignas(16) SceneDev {
   // (longest of the previous launch first) and this launch's cost per tile
   const uint32_t* tile_order;
   uint32_t* tile_cost;
+  // f64 observer event and tetrad: every pixel's initial ray is formed in
+  // f64 (the f32 trace starts from its rounding; the f64 refine from it)
+  double obs_d[3];
+  double u_t_d;
+  double right_d[4], up_d[4], arrival_d[4];
+  double tan_half_fov_d, aspect_d, sub_x_d, sub_y_d;
+  double r_observer_d;  // oblate radius of the observer
+  // near-critical test and its f64 refine (gl_scene_desc)
+  float near_critical_factor;  // <= 0: no test
+  float branch_factor;
+  float refine_rtol, refine_atol;
 };
 
-static_assert(sizeof(SceneDev) == 336, "SceneDev must stay 336 bytes");
+static_assert(sizeof(SceneDev) == 528, "SceneDev must stay 528 bytes");
 
 // Work-queue and statistics block of one persistent launch.
 struct alignas(64) QueueDev {
@@ -84,6 +95,9 @@ struct OutPlanes {
   uint32_t* branch;
   float* radiance;
   uint16_t* rgba16f;  // 4 halves per ray
+  float* event_residual;
+  float* turning_radius;
+  float* azimuth_advance;
   unsigned long long count;
 };
 

@@ -99,7 +99,7 @@ __global__ void __launch_bounds__(64) k_vjp_forward(const SceneDev* __restrict__
   }
   vb.E[idx] = R.E;
   vb.n[idx] = (R.term == kTermSurface && !overflow) ? R.accepted : 0u;
-  write_ray(R, out, idx, s, map);
+  write_ray(R, out, idx, s, map, pixel_source(px, py), no_refine(), 0u);
 }
 
 // Adjoint of the terminal (surface-crossing) step, seeded by dL/d(r, phi_s,
@@ -458,18 +458,7 @@ gl_status gl_vjp_forward(gl_trace_plan* p, uint32_t scene, double inclination,
   map.rgb = p->d_map;
   map.w = p->map_w;
   map.h = p->map_h;
-  OutPlanes planes;
-  planes.status = out->status;
-  planes.steps = out->steps;
-  planes.rejected = out->rejected;
-  planes.observable = out->observable;
-  planes.travel_time = out->travel_time;
-  planes.affine = out->affine;
-  planes.drift = out->drift;
-  planes.branch = out->branch;
-  planes.radiance = out->radiance;
-  planes.rgba16f = out->rgba16f;
-  planes.count = out->count;
+  const OutPlanes planes = to_planes(*out);
   plan_mark(p, 0, hs_);
   hipLaunchKernelGGL(k_vjp_forward, dim3(st->tiles), dim3(64), 0, hs_, p->d_scenes, scene, planes,
                      map, vb);

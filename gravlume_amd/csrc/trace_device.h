@@ -110,11 +110,20 @@ GL_HD void dense_slopes(T th, T w[7]) {
     w[i] = d[i][0] + th * (T(2) * d[i][1] + th * (T(3) * d[i][2] + th * (T(4) * d[i][3])));
 }
 
+// Without a horizon, rays that come within this oblate radius of the ring
+// singularity finish in f64: binary32 cannot resolve the approach to the
+// singularity guard (r^4 + a^2 z^2 = 2^-40), where the oblate radius comes
+// from the cancellation x^2 + y^2 + z^2 - a^2, and its step control
+// underflows before the guard is reached.
+constexpr float kRingDeferRadius = 0.25f;
+
 // Scene constants in the working precision.
 template <class T>
 struct SceneT {
   T mass, spin, spin2, charge2, chart;
-  T r_capture, r_horizon, r_escape, disk_in, disk_out;
+  // r_defer: f32 rays below it finish in f64 (1.05 r_+, or for a spacetime
+  // without a horizon kRingDeferRadius around the ring singularity)
+  T r_capture, r_defer, r_escape, disk_in, disk_out;
   T band, rtol, atol, h_min, h_max, h_max_per_r, tie_tol, drift_limit, guard;
   T safety, min_factor, max_factor;  // step controller (policy.rs:160-173)
   uint32_t max_accepted, max_rejects, options, reproject_interval;
@@ -131,7 +140,7 @@ GL_HD SceneT<float> scene_f32(const SceneDev& s) {
   c.charge2 = s.charge2;
   c.chart = s.chart;
   c.r_capture = s.r_capture;
-  c.r_horizon = s.r_horizon;
+  c.r_defer = s.r_horizon > 0.0f ? 1.05f * s.r_horizon : kRingDeferRadius;
   c.r_escape = s.r_escape;
   c.disk_in = s.disk_in;
   c.disk_out = s.disk_out;
@@ -162,7 +171,7 @@ GL_HD SceneT<double> scene_f64(const SceneDev& s) {
   c.charge2 = s.charge2_d;
   c.chart = s.chart;
   c.r_capture = s.r_capture_d;
-  c.r_horizon = s.r_horizon_d;
+  c.r_defer = s.r_horizon_d > 0.0 ? 1.05 * s.r_horizon_d : double(kRingDeferRadius);
   c.r_escape = s.r_escape;
   c.disk_in = s.disk_in_d;
   c.disk_out = s.disk_out_d;
@@ -213,6 +222,10 @@ enum : uint32_t {
   kOptF64Horizon = 2u,
   kOptF64DiskRoot = 4u,
   kOptEscapeMap = 8u,
+  kOptRefine = 16u,
+  // set by a kernel whose caller asked for the turning-radius plane: every
+  // inward-to-outward turning is then localized on the dense output
+  kOptTurningRadius = 1u << 30,
 };
 
 // Per-ray state. Position and coordinate time / affine parameter carry a
@@ -228,6 +241,15 @@ struct RayT {
   T lam, clam;
   T lz0, q0;
   T d_null, d_lz, d_q;
+  // Smallest inward-to-outward turning radius, -1 if none: localized on the
+  // dense output with kOptTurningRadius, else the smaller endpoint radius of
+  // the turning step (enough for the near-critical test's proximity check).
+  T r_turn;
+  // The near-critical test (near_critical.h): r_m, the radius of the minimum
+  // of the ray's radial potential (0: no test), and d_crit, the largest
+  // relative drift sampled at r in [r_m, 2 r_m], where the capture/scatter
+  // decision is taken.
+  T r_m, d_crit;
   uint32_t accepted, rejected, consec;
   uint32_t turnings, crossings;
   int32_t winding;
@@ -439,6 +461,9 @@ GL_HD void init_ray(const SceneT<T>& sc, RayT<T>& R, T x, T y, T z, T cx, T cy, 
   R.candidates = 0;
   R.flags = 0;
   R.polar_side = polar_side;
+  R.r_turn = T(-1);
+  R.r_m = T(0);
+  R.d_crit = T(0);
   if (!refresh_derivative<T>(sc, R)) {
     R.term = kTermFailure;
     R.flags |= kFlagNonFinite;
@@ -454,6 +479,7 @@ GL_HD void init_ray(const SceneT<T>& sc, RayT<T>& R, T x, T y, T z, T cx, T cy, 
   d.dz = R.k[3];
   R.d_null = null_residual<T>(d, px, py, pz, E);
   R.armed = arm_events<T>(sc, R.r, z, 0u);
+  R.d_crit = R.d_null;
 }
 
 // Dense position at fraction th of the step from (x0, y0, z0).
@@ -498,7 +524,7 @@ GL_HD T radius_root(const SC& sc, const T K[7][7], T h, T x0, T y0, T z0, T g0, 
   T lo = T(0), hi = T(1), glo = g0, ghi = g1;
   int side = 0;
   T th = T(0.5);
-  for (int it = 0; it < 24; ++it) {
+  for (int it = 0; it < (sizeof(T) >= 8 ? 48 : 24); ++it) {
     T den = ghi - glo;
     th = den != T(0) ? lo - glo * (hi - lo) * gl_rcp<T>(den) : T(0.5) * (lo + hi);
     if (!(th > lo && th < hi)) th = T(0.5) * (lo + hi);
@@ -518,7 +544,9 @@ GL_HD T radius_root(const SC& sc, const T K[7][7], T h, T x0, T y0, T z0, T g0, 
       if (side == 1) glo *= T(0.5);
       side = 1;
     }
-    if ((hi - lo) * gl_abs<T>(h) <= T(1e-6) * gl_max<T>(T(1), target)) break;
+    // bracket in affine parameter: 1e-6 M (per unit of radius beyond 1 M)
+    // in f32, 1e-12 M in f64
+    if ((hi - lo) * gl_abs<T>(h) <= T(sizeof(T) >= 8 ? 1e-12 : 1e-6) * gl_max<T>(T(1), target)) break;
   }
   return gl_abs<T>(glo) <= gl_abs<T>(ghi) ? lo : hi;
 }
@@ -588,6 +616,65 @@ GL_HD U z_root(const T K[7][7], T h, T z0, T z1) {
   return th;
 }
 
+// The radial turning inside an accepted step: the root of the radial
+// traversal velocity dir * grad r . dx/dlambda on the dense output, with
+// dx/dlambda(theta) = sum_i w_i'(theta) K_i (geodesic.rs:330-359 localizes
+// the same sign change of the radial velocity). The endpoint velocities
+// v0, v1 (traversal sign applied) bracket it. Illinois false position to a
+// bracket of a quarter of the tie tolerance in affine parameter; returns the
+// bracket [lo, hi] and the radius at the last point evaluated inside it.
+template <class T, class SC>
+GL_HD void turning_root(const SC& sc, const T K[7][7], T h, T x0, T y0, T z0, T v0, T v1, T dir,
+                        T& lo_out, T& hi_out, T& r_out) {
+  T lo = T(0), hi = T(1), glo = v0, ghi = v1;
+  int side = 0;
+  T r_at = T(0);
+  const T stop = T(0.25) * sc.tie_tol;
+  const T hmag = gl_abs<T>(h);
+  for (int it = 0; it < 32; ++it) {
+    const T den = ghi - glo;
+    T th = den != T(0) ? lo - glo * (hi - lo) * gl_rcp<T>(den) : T(0.5) * (lo + hi);
+    if (!(th > lo && th < hi)) th = T(0.5) * (lo + hi);
+    T x, y, z;
+    dense_position<T>(K, h, x0, y0, z0, th, x, y, z);
+    T w[7];
+    dense_slopes<T>(th, w);
+    T vx = w[0] * K[0][1], vy = w[0] * K[0][2], vz = w[0] * K[0][3];
+#pragma unroll
+    for (int i = 2; i < 7; ++i) {
+      vx += w[i] * K[i][1];
+      vy += w[i] * K[i][2];
+      vz += w[i] * K[i][3];
+    }
+    const Geom<T> g = geometry<T>(x, y, z, sc.mass, sc.spin, sc.spin2, sc.charge2, sc.chart);
+    const T gv = dir * (g.grx * vx + g.gry * vy + g.grz * vz);
+    r_at = g.r;
+    if (gv == T(0)) {
+      lo = hi = th;
+      break;
+    }
+    if ((gv < T(0)) == (glo < T(0))) {
+      lo = th;
+      glo = gv;
+      if (side == -1) ghi *= T(0.5);
+      side = -1;
+    } else {
+      hi = th;
+      ghi = gv;
+      if (side == 1) glo *= T(0.5);
+      side = 1;
+    }
+    if ((hi - lo) * hmag <= stop) break;
+  }
+  lo_out = lo;
+  hi_out = hi;
+  r_out = r_at;
+}
+
+template <class T>
+GL_HD void note_turning_radius(RayT<T>& R, T r) {
+  R.r_turn = R.r_turn < T(0) ? r : gl_min<T>(R.r_turn, r);
+}
 
 template <class T>
 GL_HD T step_cap(const SceneT<T>& sc, T r) {
@@ -609,6 +696,12 @@ GL_HD void update_drift(const SceneT<T>& sc, RayT<T>& R, const Geom<T>& g, const
   R.d_null = gl_max<T>(R.d_null, nr);
   R.d_lz = gl_max<T>(R.d_lz, dl);
   R.d_q = gl_max<T>(R.d_q, dq);
+  // the near-critical budget counts the drift where the capture/scatter
+  // decision is taken, r in [r_m, 2 r_m]: beyond, the invariants' own
+  // evaluation (terms of size r^2 p^2 cancelling to Carter's constant) is
+  // the larger error, and below r_m a captured ray is already decided
+  if (g.r >= R.r_m && g.r <= T(2) * R.r_m)
+    R.d_crit = gl_max<T>(R.d_crit, gl_max<T>(nr, gl_max<T>(dl, dq)));
   if (!(R.d_null <= sc.drift_limit && R.d_lz <= sc.drift_limit && R.d_q <= sc.drift_limit))
     R.flags |= kFlagDrift;
 }
@@ -868,11 +961,18 @@ GL_HD void step(const SceneT<T>& sc, RayT<T>& R, T omega, bool reproject = false
     Deriv<T> dl;
     Geom<T> gl_;
     bool okl = eval<T>(sc, ys[0], ys[1], ys[2], ys[3], ys[4], ys[5], R.E, dl, gl_);
-    if (turning && okl) {
-      // The turning commits only strictly before the terminal fraction: the
-      // radial traversal velocity at the terminal has already changed sign.
-      T vt = dir * radial_rate<T>(gl_, dl);
-      if ((v0 < T(0) && vt >= T(0)) || (v0 > T(0) && vt <= T(0))) R.turnings += 1;
+    if (turning) {
+      // The turning commits only when its root bracket lies strictly before
+      // the terminal fraction; a terminal inside the bracket leaves the
+      // order of the two unresolved (geodesic_integration.wgsl:298-318).
+      T tlo, thi, rt;
+      turning_root<T>(sc, K, h, y0[0], y0[1], y0[2], v0, v1, dir, tlo, thi, rt);
+      if (thi < best) {
+        R.turnings += 1;
+        if (v0 < T(0)) note_turning_radius<T>(R, rt);
+      } else if (tlo < best) {
+        R.flags |= kFlagTurningOrder;
+      }
     }
     if (zcross && thz < best) R.crossings += 1;
     update_winding<T>(R, y0[0], y0[1], ys[0], ys[1]);
@@ -897,7 +997,17 @@ GL_HD void step(const SceneT<T>& sc, RayT<T>& R, T omega, bool reproject = false
   }
 
   // Non-terminal commit.
-  if (turning) R.turnings += 1;
+  if (turning) {
+    R.turnings += 1;
+    if (v0 < T(0)) {
+      T rt = gl_min<T>(r0, r1);
+      if (sc.options & kOptTurningRadius) {
+        T tlo, thi;
+        turning_root<T>(sc, K, h, y0[0], y0[1], y0[2], v0, v1, dir, tlo, thi, rt);
+      }
+      note_turning_radius<T>(R, rt);
+    }
+  }
   if (zcross) R.crossings += 1;
   kahan_add<T>(R.t, R.ct, dtinc);
   kahan_add<T>(R.lam, R.clam, h);
@@ -920,11 +1030,17 @@ GL_HD void step(const SceneT<T>& sc, RayT<T>& R, T omega, bool reproject = false
   R.armed = arm_events<T>(sc, r1, z1, R.armed);
   if (reproject) {
     reproject_null<T>(sc, R, R.px, R.py, R.pz);
+    // The radial rate the next step compares against stays the committed
+    // one: close to a turning point the radial velocity is tiny, and the
+    // re-projected momentum can flip its sign without any step, so a turning
+    // would pass between two steps unseen.
+    const T rdot_committed = R.rdot;
     if (!refresh_derivative<T>(sc, R)) {
       R.term = kTermFailure;
       R.flags |= kFlagNonFinite;
       return;
     }
+    R.rdot = rdot_committed;
   }
   // Step control of policy.rs:160-173 (safety 0.9, factor in [0.2, 5] by
   // default: the scene's safety, min_factor, max_factor) with

@@ -177,6 +177,13 @@ class Scene:
     subpixel: tuple[float, float] = (0.5, 0.5)
     options: int = (_capi.OPT_F64_DISK_ROOT | _capi.OPT_F64_HORIZON_STEPS)
     capture_certificate: bool = False  # interval radial-capture certificate
+    # near-critical test (0: the library's 16; negative: off) and the f64
+    # refine of near-critical rays at refine_rtol / refine_atol (0: 1e-12 / 1e-14)
+    near_critical_factor: float = 0.0
+    branch_factor: float = 0.0  # 0: the library's 4096
+    refine: bool = False
+    refine_rtol: float = 0.0
+    refine_atol: float = 0.0
     escape_map: bool = True
     extra: dict = field(default_factory=dict)
 
@@ -216,6 +223,8 @@ class Scene:
             opts |= _capi.OPT_CAPTURE_CERTIFICATE
         if self.escape_map:
             opts |= _capi.OPT_ESCAPE_MAP
+        if self.refine:
+            opts |= _capi.OPT_NEAR_CRITICAL_REFINE
         d.options = opts
         x, y, z = self.observer_xyz()
         d.observer_txyz[:] = [0.0, x, y, z]
@@ -256,6 +265,10 @@ class Scene:
         d.arming_band = self.arming_band
         d.drift_limit = self.drift_limit
         d.reproject_interval = self.reproject_interval
+        d.near_critical_factor = self.near_critical_factor
+        d.branch_factor = self.branch_factor
+        d.refine_rtol = self.refine_rtol
+        d.refine_atol = self.refine_atol
         return d
 
 
@@ -303,16 +316,14 @@ def near_critical_scene(**kw) -> Scene:
     return Scene(**base)
 
 
-def uneven_scenes(count: int = 256, seed: int = 20261206, **kw) -> list[Scene]:
-    """Seed-drawn scenes of uneven size: a ~ U[0, 0.998], Q ~ U[0, 0.9 sqrt(1 - a^2)],
-    inclination ~ U[5, 89] deg, r_obs ~ U[30, 200] M, side in {128..1024},
-    FOV ~ U[8, 30] deg; disk [r_ISCO, 20 M], escape at 1000 M. A draw that
-    fails the scene checks (gl_scene_validate) is discarded and drawn again."""
+def uneven_scene_draws(seed: int = 20261206, **kw):
+    """The stream of seed-drawn scenes of uneven size: a ~ U[0, 0.998],
+    Q ~ U[0, 0.9 sqrt(1 - a^2)], inclination ~ U[5, 89] deg,
+    r_obs ~ U[30, 200] M, side in {128..1024}, FOV ~ U[8, 30] deg; disk
+    [r_ISCO, 20 M], escape at 1000 M. An endless generator of candidates."""
     rng = np.random.default_rng(seed)
     sides = [128, 256, 512, 768, 1024]
-    lib = _capi.load()
-    out = []
-    while len(out) < count:
+    while True:
         a = float(rng.uniform(0.0, 0.998))
         q = float(rng.uniform(0.0, math.sqrt(1.0 - a * a) * 0.9))
         side = int(rng.choice(sides))
@@ -321,9 +332,22 @@ def uneven_scenes(count: int = 256, seed: int = 20261206, **kw) -> list[Scene]:
                   fov_deg=float(rng.uniform(8.0, 30.0)), disk=(None, 20.0), escape_map=False)
         for k, v in kw.items():
             setattr(s, k, v)
+        yield s
+
+
+def uneven_scenes(count: int = 256, seed: int = 20261206, **kw) -> list[Scene]:
+    """The first `count` draws of uneven_scene_draws that pass the scene
+    checks (gl_scene_validate, which accepts exactly the observers the
+    reference's frame construction accepts). For the default seed no draw
+    of the first 256 is refused, so this is the plain draw sequence."""
+    lib = _capi.load()
+    out = []
+    for s in uneven_scene_draws(seed, **kw):
+        if len(out) >= count:
+            break
         d = s.to_desc()
         if lib.gl_scene_validate(C.byref(d)) != 0:
-            continue  # e.g. an observer frame outside the 1e-12 Gram budget: draw again
+            continue
         out.append(s)
     return out
 

@@ -42,7 +42,8 @@ class TileRegion:
 class TraceTarget:
     """Caller-owned structure-of-arrays outputs for n rays on the device."""
 
-    def __init__(self, rays: int, device: str | torch.device = "cuda", scene: int | None = None):
+    def __init__(self, rays: int, device: str | torch.device = "cuda", scene: int | None = None,
+                 turning_radius: bool = False):
         dev = torch.device(device)
         self.rays = rays
         self.scene = scene  # the image scene it was created for (None: a batch)
@@ -57,12 +58,21 @@ class TraceTarget:
         self.radiance = torch.zeros(3 * rays, dtype=torch.float32, device=dev)
         # scene-linear RGBA16F texels, 4 halves per ray (R, G, B, A)
         self.rgba16f = torch.zeros(4 * rays, dtype=torch.int16, device=dev)
+        self.event_residual = torch.zeros(rays, dtype=torch.float32, device=dev)
+        self.azimuth_advance = torch.zeros(rays, dtype=torch.float32, device=dev)
+        # the turning localization runs only for a target that holds this plane
+        self.turning_radius = (torch.zeros(rays, dtype=torch.float32, device=dev)
+                               if turning_radius else None)
+
+    PLANES = ("status", "steps", "rejected", "observable", "travel_time", "affine", "drift",
+              "branch", "radiance", "rgba16f", "event_residual", "turning_radius",
+              "azimuth_advance")
 
     def c_struct(self) -> _capi.RayOutputsC:
         o = _capi.RayOutputsC()
-        for name in ("status", "steps", "rejected", "observable", "travel_time", "affine",
-                     "drift", "branch", "radiance", "rgba16f"):
-            setattr(o, name, getattr(self, name).data_ptr())
+        for name in self.PLANES:
+            t = getattr(self, name)
+            setattr(o, name, t.data_ptr() if t is not None else None)
         o.count = self.rays
         return o
 
@@ -75,6 +85,13 @@ class TraceTarget:
             "termination": (st & 15).astype(np.int32),
             "candidates": ((st >> 4) & 15).astype(np.int32),
             "flags": ((st >> 8) & 255).astype(np.int32),
+            # for Uncertain rays the class the trace reached before the downgrade
+            "provisional": ((st >> _capi.STATUS_PROVISIONAL_SHIFT) & 15).astype(np.int32),
+            "near_critical": (st & _capi.STATUS_NEAR_CRITICAL) != 0,
+            "refined": (st & _capi.STATUS_REFINED) != 0,
+            "branch_unsettled": (st & _capi.STATUS_BRANCH_UNSETTLED) != 0,
+            "event_residual": self.event_residual.cpu().numpy(),
+            "azimuth_advance": self.azimuth_advance.cpu().numpy(),
             "steps": self.steps.cpu().numpy().view(np.uint32).astype(np.int64),
             "rejected": self.rejected.cpu().numpy().view(np.uint32).astype(np.int64),
             "observable": self.observable.cpu().numpy().reshape(3, n).T.copy(),
@@ -88,6 +105,12 @@ class TraceTarget:
             "radiance": self.radiance.cpu().numpy().reshape(3, n).T.copy(),
             "rgba16f": self.rgba16f.cpu().numpy().view(np.float16).reshape(n, 4).copy(),
         }
+        if self.turning_radius is not None:
+            out["turning_radius"] = self.turning_radius.cpu().numpy()
+        # the definite class of every ray: its own class, or for an Uncertain
+        # ray the class it reached before the downgrade
+        out["class_reached"] = np.where(out["termination"] == _capi.TERM_UNCERTAIN,
+                                        out["provisional"], out["termination"])
         return out
 
 
@@ -324,6 +347,10 @@ class FrameGraph:
             self._handle = None
 
 
+class GeodesicBatchError(ValueError):
+    """Invalid batch configuration (GeodesicBatchError, batch.rs:7-17)."""
+
+
 class GeodesicBatch:
     """Ordered batch of geodesic traces on the GPU (GeodesicBatch::trace_ordered).
 
@@ -331,11 +358,21 @@ class GeodesicBatch:
     (t, x, y, z, p_t, p_x, p_y, p_z) in f64 and affine directions (+1 forward,
     -1 backward); the spacetime, policy and events are those of scene
     `scene` of the pipeline. Outcome i belongs to request i.
+
+    ``threads`` mirrors GeodesicBatch::new(NonZeroUsize) (batch.rs:19-29): it
+    must lie in [1, 256] as there, and is otherwise not used, since one GPU
+    lane traces each request.
     """
 
-    def __init__(self, pipeline: TracePipeline, scene: int = 0):
+    MAX_THREADS = 256
+
+    def __init__(self, pipeline: TracePipeline, scene: int = 0, threads: int = 1):
+        if not isinstance(threads, int) or threads < 1 or threads > self.MAX_THREADS:
+            raise GeodesicBatchError(
+                f"thread count must lie in [1, {self.MAX_THREADS}], got {threads!r}")
         self.pipeline = pipeline
         self.scene = scene
+        self.threads = threads
 
     def trace_ordered(self, states: np.ndarray, directions: np.ndarray, stream=None,
                       persistent: bool = False, config: dict | None = None) -> dict:
@@ -345,7 +382,8 @@ class GeodesicBatch:
         dirs = np.ascontiguousarray(directions, dtype=np.float32).reshape(-1)
         if dirs.shape[0] != n:
             raise ValueError("one direction per state is required")
-        target = TraceTarget(n)
+        # ReferenceOutcome carries the turning radius (outcome.rs:425-436)
+        target = TraceTarget(n, turning_radius=True)
         if n == 0:
             return target.to_host()
         d_states = torch.from_numpy(states).cuda()

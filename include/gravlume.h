@@ -46,6 +46,32 @@ enum {
   GL_TERM_REJECT_EXHAUSTION = 8
 };
 
+/* The status word of a ray (gl_ray_outputs.status):
+ *   bits 0-3   termination class (GL_TERM_*)
+ *   bits 4-7   event candidates (GL_EVENT_*)
+ *   bits 8-15  numerical flags (GL_FLAG_*)
+ *   bits 16-19 for GL_TERM_UNCERTAIN: the definite class the trace reached
+ *              before the downgrade (0 otherwise)
+ *   bit  20    GL_STATUS_NEAR_CRITICAL: the ray's distance from the critical
+ *              curve (the radial-potential margin of its constants of
+ *              motion) lies below the trace's own error budget
+ *   bit  21    GL_STATUS_REFINED: the ray was traced again in f64 at the
+ *              scene's refine tolerance (GL_OPT_NEAR_CRITICAL_REFINE)
+ *   bit  22    GL_STATUS_BRANCH_UNSETTLED: the margin lies below
+ *              branch_factor times the error budget, so the branch key
+ *              (turnings, crossings, winding), the travel time and the
+ *              azimuth advance, which grow with the time spent near the
+ *              critical orbit, are not settled; the class may still be
+ * A ray is GL_TERM_UNCERTAIN when its terminal events tie
+ * (GL_FLAG_TIED_EVENTS), a radial turning lies inside the tie window of the
+ * terminal (GL_FLAG_TURNING_ORDER), an invariant drift exceeds the scene
+ * budget (GL_FLAG_DRIFT_BUDGET) (geodesic_integration.wgsl:324-330), or it is
+ * near-critical and not refined (docs/validation.md:287). */
+#define GL_STATUS_PROVISIONAL_SHIFT 16
+#define GL_STATUS_NEAR_CRITICAL (1u << 20)
+#define GL_STATUS_REFINED (1u << 21)
+#define GL_STATUS_BRANCH_UNSETTLED (1u << 22)
+
 /* Event-candidate bits (trace_protocol.wgsl:95-98, geodesic.rs:616-634). */
 enum {
   GL_EVENT_SINGULARITY = 1u,
@@ -77,7 +103,12 @@ enum {
                                       with GL_FLAG_CERTIFIED_CAPTURE */
   GL_OPT_F64_HORIZON_STEPS = 2u,   /* f64 DP5 steps inside 1.05 r_+            */
   GL_OPT_F64_DISK_ROOT = 4u,       /* f64 equatorial root solve + observables  */
-  GL_OPT_ESCAPE_MAP = 8u           /* look escape directions up in the map     */
+  GL_OPT_ESCAPE_MAP = 8u,          /* look escape directions up in the map     */
+  GL_OPT_NEAR_CRITICAL_REFINE = 16u /* near-critical rays (bit 20 of the status)
+                                      are traced again from f64 initial data in
+                                      f64 at refine_rtol / refine_atol; they
+                                      stay GL_TERM_UNCERTAIN only if the refined
+                                      trace is still near-critical */
 };
 
 /* Surface emission and transport models (EquatorialEmissionModel,
@@ -139,8 +170,25 @@ typedef struct gl_scene_desc {
   double event_tie_tolerance; /* M */
   double arming_band;         /* M */
   double drift_limit;         /* relative invariant drift budget */
-  uint32_t reproject_interval; /* f64 null re-projection every N accepted steps (0 off) */
+  uint32_t reproject_interval; /* f64 null re-projection every N DP5 attempts
+                                  (accepted or rejected) of a ray; 0 off */
   uint32_t reserved;
+  /* Near-critical test: a ray that came within 1.25 r_m of the radius r_m
+   * where its radial potential R(r) is smallest over [r_+, r_start] is
+   * near-critical when |R(r_m)| / (sum of the magnitudes of R's two terms at
+   * r_m) <= near_critical_factor * (the trace's error budget: the largest of
+   * its relative null, L_z and Carter drifts sampled at r in [r_m, 2 r_m],
+   * and for an f32 trace the f32 rounding of its initial constants). 0
+   * selects 16; negative disables the test (and the branch test). Scenes
+   * without a horizon are never near-critical. */
+  double near_critical_factor;
+  /* the same test for the branch key and the orbit-dependent observables
+   * (GL_STATUS_BRANCH_UNSETTLED); 0 selects 4096, must be >= the
+   * near-critical factor */
+  double branch_factor;
+  /* f64 refine policy of GL_OPT_NEAR_CRITICAL_REFINE (which re-traces every
+   * ray with bit 20 or 22); 0 selects 1e-12 / 1e-14 */
+  double refine_rtol, refine_atol;
 } gl_scene_desc;
 
 /* Rectangle of 8x8 tiles of one scene (TileRegion, trace.rs:799-857). */
@@ -171,7 +219,18 @@ typedef struct gl_tile_region {
  *   rgba16f     the scene-linear RGBA16F texel, 4 halves per ray in R, G, B,
  *               A order (lensing_preview.wgsl:64-82): surface (radiance, 2),
  *               escape (map RGB, 1), horizon (0, 0, 0, 0), anything else a
- *               failure colour with alpha -termination */
+ *               failure colour with alpha -termination
+ *   event_residual  the localized event function re-evaluated at the
+ *               terminal state, (value - event surface) / max(1, |surface|):
+ *               r - r_escape or r - r_capture, z on the disk, the singularity
+ *               measure minus its guard; 0 for a terminal without an event
+ *               (GeometricSample.event_residual, trace_protocol.wgsl:76)
+ *   turning_radius  the smallest radius at a localized inward-to-outward
+ *               radial turning, -1 when the ray made none
+ *               (ReferenceOutcome::turning_radius_m, outcome.rs:425-436)
+ *   azimuth_advance the signed azimuth swept from the initial to the terminal
+ *               state, the sum of wrapped atan2(y, x) differences between
+ *               committed states (ReferenceOutcome::azimuth_advance_rad) */
 typedef struct gl_ray_outputs {
   uint32_t* status;
   uint32_t* steps;
@@ -183,6 +242,9 @@ typedef struct gl_ray_outputs {
   uint32_t* branch;
   float* radiance;   /* 3 planes */
   uint16_t* rgba16f; /* 4 halves per ray, interleaved */
+  float* event_residual;
+  float* turning_radius;  /* the turning localization runs only when given */
+  float* azimuth_advance;
   size_t count;      /* number of rays the planes hold */
 } gl_ray_outputs;
 

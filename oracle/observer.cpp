@@ -73,7 +73,13 @@ bool project_and_normalize(const double g[4][4], const FV& u, const FV& seed, co
 }
 
 // best_coordinate_axis (observer.rs:319-352): the candidate with the largest
-// spatial max-norm; on ties the later axis wins (Iterator::max_by).
+// spatial max-norm; on ties the later axis wins (Iterator::max_by). Two axes
+// whose remainders normalize to the same direction tie exactly in exact
+// arithmetic, but the one with the tiny remainder (an axis nearly inside the
+// span already projected out) carries its cancellation error into the score,
+// which then decides the pick by a last-bit accident and, when it goes to
+// that axis, the frame fails the 1e-12 Gram budget. Scores within a relative
+// 1e-9 are therefore taken as the tie they are in exact arithmetic.
 bool best_coordinate_axis(const double g[4][4], const FV& u, const FV* ortho, int n_ortho,
                           FV* out) {
   bool found = false;
@@ -85,7 +91,7 @@ bool best_coordinate_axis(const double g[4][4], const FV& u, const FV* ortho, in
     if (!project_and_normalize(g, u, seed, ortho, n_ortho, &cand)) continue;
     double score = 0.0;
     for (int i = 1; i < 4; ++i) score = std::fmax(score, std::fabs(cand.v[i]));
-    if (!found || score >= best) {
+    if (!found || score >= best * (1.0 - 1.0e-9)) {
       best = score;
       *out = cand;
       found = true;
