@@ -127,23 +127,33 @@ def test_product_isco_matches_oracle(oracle):
 
 
 def test_scene_validate_needs_no_device_and_matches_the_oracle_frame_check(oracle):
-    """gl_scene_validate runs the plan's scene checks on the host. An observer
-    frame that exceeds the 1e-12 Gram budget is rejected by it exactly as by
-    the oracle's restatement of the reference frame construction."""
+    """gl_scene_validate runs the plan's scene checks on the host, with the
+    oracle's verdict on the observer frame: a draw whose right-leg axes tie
+    in their max-norm score (an axis nearly inside the span of the sight and
+    up legs normalizes to the same direction as its neighbour) builds in both
+    within the 1e-12 Gram budget; an observer inside the ergosphere (g_tt >= 0)
+    is refused by both."""
     import math
     from gravlume_amd import _capi, scene as S
     lib = _capi.load()
     good = S.small_cpu_scene(escape_map=False)
     assert lib.gl_scene_validate(C.byref(good.to_desc())) == 0
-    bad = S.Scene(spin=0.08186844760218794, charge=0.7823507233737966,
+    tie = S.Scene(spin=0.08186844760218794, charge=0.7823507233737966,
                   inclination_deg=74.81317707591471, observer_radius=196.28397290333865,
                   width=512, height=512, fov_deg=20.0, disk=(None, 20.0), escape_map=False)
-    assert lib.gl_scene_validate(C.byref(bad.to_desc())) == 2
-    assert b"Gram" in lib.gl_last_error()
+    assert lib.gl_scene_validate(C.byref(tie.to_desc())) == 0, lib.gl_last_error()
+    o = oracle.observation(1.0, tie.spin, tie.charge, float(tie.chart), [0.0, *tie.observer_xyz()],
+                           [0, 0, 0, 0], [0, 0, 1], 1.0, tie.width, tie.height,
+                           math.radians(tie.fov_deg))
+    assert o.gram_residual <= 1e-12
+    ergo = S.Scene(spin=0.9, charge=0.0, inclination_deg=90.0, observer_radius=1.9, width=64,
+                   height=64, fov_deg=20.0, disk=None, escape_map=False)
+    assert lib.gl_scene_validate(C.byref(ergo.to_desc())) == _capi.GL_ERR_DOMAIN
+    assert b"g_tt" in lib.gl_last_error()
     with pytest.raises(ValueError):
-        oracle.observation(1.0, bad.spin, bad.charge, float(bad.chart), [0.0, *bad.observer_xyz()],
-                           [0, 0, 0, 0], [0, 0, 1], 1.0, bad.width, bad.height,
-                           math.radians(bad.fov_deg))
+        oracle.observation(1.0, ergo.spin, ergo.charge, float(ergo.chart),
+                           [0.0, *ergo.observer_xyz()], [0, 0, 0, 0], [0, 0, 1], 1.0, 64, 64,
+                           math.radians(ergo.fov_deg))
 
 
 def test_step_controller_factors_are_validated():

@@ -6,6 +6,9 @@
   trace of the same scene, every pixel keeps its termination class and event
   candidates, traced pixels are bit-identical, and reconstructed escape
   directions lie within the reference's 3.82e-4 chord budget.
+* Reconstructed pixels against the f64 oracle: on the default profile, the
+  blended escape direction of 512 seeded reconstructed pixels lies within
+  the same 3.82e-4 chord of the oracle's traced direction (rtol 1e-10).
 * Shadow-edge coverage (shadow_coverage.wgsl:1-136): the edge list is stable
   under repetition, only edge texels change, and the coverage of sampled edge
   pixels equals the f64 oracle's count of escaping sub-samples wherever the
@@ -101,12 +104,12 @@ def _texel(rgb, d, margin=1e-3):
 
 
 def test_shadow_refinement_coverage_matches_the_oracle(gl, oracle):
-    """The reference's default view (a = 0.8, r = 30 M, 60 deg, FOV 45 deg)
-    at 320x180 with the escape map."""
+    """The reference's default view (a = 0.8, outgoing chart, r = 30 M, 60 deg,
+    FOV 45 deg; gpu_trace_tests.rs:849-855) at 320x180 with the escape map."""
     from gravlume_amd import scene as S
     import torch
     sc = S.Scene(spin=0.8, charge=0.0, inclination_deg=60.0, observer_radius=30.0, width=320,
-                 height=180, fov_deg=45.0, chart=S.INGOING, disk=None, escape_radius=200.0)
+                 height=180, fov_deg=45.0, chart=S.OUTGOING, disk=None, escape_radius=200.0)
     pipe = gl.TracePipeline([sc])
     tgt = pipe.create_target(0)
     pipe.encode_batch(tgt)
@@ -184,3 +187,29 @@ def test_shadow_refinement_coverage_matches_the_oracle(gl, oracle):
         ok += 1
     print("colour-checked edge pixels", ok)
     assert ok >= 10
+
+
+def test_reconstructed_pixels_match_the_oracle(gl, oracle):
+    from gravlume_amd import scene as S
+    import torch
+    sc = _profile(S, 0.8, 0.0, 30.0, math.pi / 4)
+    pipe = gl.TracePipeline([sc])
+    acc = pipe.create_target(0)
+    pipe.encode_accelerated(acc)
+    torch.cuda.synchronize()
+    h = acc.to_host()
+    rec = np.flatnonzero((h["flags"] & 128) != 0)
+    assert rec.size > 10000
+    idx = np.sort(np.random.default_rng(23).choice(rec, 512, replace=False))
+    obs = oracle.observation(1.0, sc.spin, sc.charge, float(sc.chart), [0.0, *sc.observer_xyz()],
+                             [0, 0, 0, 0], [0, 0, 1], 1.0, sc.width, sc.height,
+                             math.radians(sc.fov_deg))
+    ev = oracle.events(sc.escape_radius, None, horizon_offset=sc.horizon_offset)
+    sxy = oracle.sight_planes(obs, idx % sc.width, idx // sc.width)
+    outs, st = oracle.observation_trace_ordered(obs, oracle.config_policy(1e-10, 1e-12), ev, sxy,
+                                                None, 16)
+    A = oracle.outcomes_to_arrays(outs)
+    assert np.all(A["termination"] == 2)
+    chord = np.linalg.norm(h["observable"][idx].astype(np.float64) - A["escape_direction"], axis=1)
+    print("reconstructed vs oracle chord: max", chord.max(), "p99", np.quantile(chord, 0.99))
+    assert chord.max() <= CHORD_BUDGET

@@ -20,8 +20,10 @@ def gl():
 
 
 def _default_view(S, **kw):
+    # the reference's default view (gpu_trace_tests.rs:849-855: a = 0.8, the
+    # outgoing chart, r = 30 M at 60 deg) with a [6, 20] M surface
     base = dict(spin=0.8, charge=0.0, inclination_deg=60.0, observer_radius=30.0, width=64,
-                height=36, fov_deg=45.0, chart=S.INGOING, disk=(6.0, 20.0), escape_radius=200.0,
+                height=36, fov_deg=45.0, chart=S.OUTGOING, disk=(6.0, 20.0), escape_radius=200.0,
                 escape_map=False, emission="bolometric")
     base.update(kw)
     return S.Scene(**base)
@@ -56,10 +58,9 @@ def test_empty_requests_are_no_ops(gl):
 @pytest.mark.parametrize("wh", [(1, 1), (1, 9), (9, 1), (7, 5)])
 def test_smallest_viewports_trace_alike_in_every_dispatch(gl, oracle, wh):
     """Viewports of a single pixel, a single row or column, and a partial
-    tile: the plain and whole-tile dispatches agree bit for bit, the
-    lane-refill dispatch in class, and every ray's class matches the oracle's
-    except where the oracle's own class changes under a 16x tighter
-    tolerance."""
+    tile: the plain, whole-tile and lane-refill dispatches agree bit for bit,
+    and every definite class matches the oracle's except where the oracle's
+    own class changes under a 16x tighter tolerance."""
     import math
     from gravlume_amd import scene as S
     import torch
@@ -78,7 +79,7 @@ def test_smallest_viewports_trace_alike_in_every_dispatch(gl, oracle, wh):
     ha, hb, hc = a.to_host(), b.to_host(), c.to_host()
     for k in ha:
         assert np.array_equal(np.asarray(ha[k]), np.asarray(hc[k])), k
-    assert np.array_equal(ha["termination"], hb["termination"])
+        assert np.array_equal(np.asarray(ha[k]), np.asarray(hb[k])), k
     obs = oracle.observation(1.0, sc.spin, sc.charge, float(sc.chart), [0.0, *sc.observer_xyz()],
                              [0, 0, 0, 0], [0, 0, 1], 1.0, w, h, math.radians(sc.fov_deg))
     ev = oracle.events(sc.escape_radius, sc.disk_radii(), horizon_offset=sc.horizon_offset)
@@ -91,7 +92,9 @@ def test_smallest_viewports_trace_alike_in_every_dispatch(gl, oracle, wh):
     A, F = oracle.outcomes_to_arrays(ref), oracle.outcomes_to_arrays(fine)
     settled = A["termination"] == F["termination"]  # not near-critical for the oracle itself
     assert settled.sum() >= len(idx) - 1
-    assert np.array_equal(ha["termination"][settled], A["termination"][settled])
+    definite = settled & (ha["termination"] != 6)
+    assert definite.sum() >= len(idx) - 1
+    assert np.array_equal(ha["termination"][definite], A["termination"][definite])
 
 
 def test_output_planes_too_small_are_rejected(gl):

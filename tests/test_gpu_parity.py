@@ -2,10 +2,17 @@
 
 Tolerances of the f32 DP5(4) trace at rtol 1e-6 / atol 1e-8 against the f64
 oracle at rtol 1e-10 (the reference's GPU budget table, docs/validation.md
-section 5.3, is the model; the numbers below are this build's):
-  * termination class and event id: exact on every ray the oracle does not
-    flag near-critical (its class or candidates change under a 16x tighter
-    tolerance), at least 99.9% of the rest;
+section 5.3, is the model; the numbers below are this build's). On every ray
+the oracle does not flag near-critical (its class or candidates change under
+a 16x tighter tolerance):
+  * termination class and event id: exact on every ray with a definite GPU
+    class; a ray the trace cannot settle is Uncertain (class 6), at most
+    0.2% of a sample;
+  * branch key (radial turnings, equatorial crossings, azimuth winding,
+    polar side): exact on definite rays whose branch is settled (no
+    GL_STATUS_BRANCH_UNSETTLED);
+  * event residual (the event function at the localized terminal): at most
+    5e-3, the reference GPU gate (gpu_trace_tests.rs:602-605);
   * escape direction: 2e-4 rad at the 99th percentile;
   * travel time: 1e-4 relative at the 99th percentile;
   * frequency ratio g: 1e-4 relative at the 99th percentile;
@@ -19,8 +26,8 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-TOL = dict(term=0.999, esc_angle_p99=2e-4, tt_rel_p99=1e-4, g_rel_p99=1e-4, r_abs_p99=1e-3,
-           null_p99=1e-4)
+TOL = dict(uncertain=2e-3, esc_angle_p99=2e-4, tt_rel_p99=1e-4, g_rel_p99=1e-4, r_abs_p99=1e-3,
+           null_p99=1e-4, residual=5e-3)
 
 
 @pytest.fixture(scope="module")
@@ -55,17 +62,22 @@ def _angle(a, b):
 def _compare(h, A, idx, tol=TOL):
     ok = ~A["flagged"]
     gt, gc = h["termination"][idx], h["candidates"][idx]
-    agree = np.mean((gt == A["termination"])[ok])
-    agree_ev = np.mean(((gt == A["termination"]) & (gc == A["candidates"]))[ok])
-    assert agree >= tol["term"], f"termination agreement {agree}"
-    assert agree_ev >= tol["term"], f"event-id agreement {agree_ev}"
-    esc = ok & (gt == 2) & (A["termination"] == 2)
+    definite = ok & (gt != 6)
+    unc = float(np.mean(gt[ok] == 6))
+    assert unc <= tol["uncertain"], f"uncertain share {unc}"
+    bad = definite & (gt != A["termination"])
+    assert not bad.any(), ("class", idx[bad], gt[bad], A["termination"][bad])
+    bad = definite & (gc != A["candidates"])
+    assert not bad.any(), ("event id", idx[bad], gc[bad], A["candidates"][bad])
+    ev = definite & np.isin(gt, (1, 2, 3, 7))
+    assert np.all(np.abs(h["event_residual"][idx][ev]) <= tol["residual"])
+    esc = definite & (gt == 2)
     if esc.sum() >= 20:
         a = _angle(h["observable"][idx][esc], A["escape_direction"][esc])
         assert np.quantile(a, 0.99) <= tol["esc_angle_p99"], np.quantile(a, 0.99)
         tt = np.abs(h["travel_time"][idx][esc] / A["travel_time"][esc] - 1)
         assert np.quantile(tt, 0.99) <= tol["tt_rel_p99"], np.quantile(tt, 0.99)
-    srf = ok & (gt == 7) & (A["termination"] == 7)
+    srf = definite & (gt == 7)
     if srf.sum() >= 20:
         o = h["observable"][idx][srf]
         g = np.abs(o[:, 2] / A["frequency_ratio"][srf] - 1)
@@ -75,12 +87,12 @@ def _compare(h, A, idx, tol=TOL):
         dphi = np.abs((o[:, 1] - A["source_azimuth"][srf] + np.pi) % (2 * np.pi) - np.pi)
         assert np.quantile(dphi, 0.99) <= 1e-3
     assert np.quantile(h["drift"][idx][:, 0], 0.99) <= tol["null_p99"]
-    # branch key on rays whose class agrees
-    same = ok & (gt == A["termination"])
-    for key, ref in (("radial_turnings", "radial_turnings"),
-                     ("equatorial_crossings", "equatorial_crossings"),
-                     ("azimuth_winding", "azimuth_winding")):
-        assert np.mean((h[key][idx] == A[ref])[same]) >= 0.995, key
+    # the branch key, exact where the trace settles it
+    settled = definite & ~h["branch_unsettled"][idx]
+    for key in ("radial_turnings", "equatorial_crossings", "azimuth_winding", "polar_side"):
+        bad = settled & (h[key][idx] != A[key])
+        assert not bad.any(), (key, idx[bad], h[key][idx][bad], A[key][bad])
+    return dict(uncertain=unc, definite=int(definite.sum()), settled=int(settled.sum()))
 
 
 def test_small_scene_matches_oracle(gl, oracle):
@@ -111,7 +123,11 @@ def test_kerr_1080p_frame_sample_matches_oracle(gl, oracle):
     h = tgt.to_host()
     # size-independent properties of the whole frame
     assert np.all(h["termination"] > 0)
-    assert set(np.unique(h["termination"])) <= {1, 2, 7}
+    assert set(np.unique(h["termination"])) <= {1, 2, 6, 7}
+    unc = h["termination"] == 6
+    assert unc.mean() <= 1e-4  # near-critical share of the frame
+    assert np.all(h["near_critical"][unc] | (h["flags"][unc] & (8 | 16 | 32) != 0))
+    assert np.all(np.isin(h["provisional"][unc], (1, 2, 7)))
     esc = h["termination"] == 2
     n = np.linalg.norm(h["observable"][esc], axis=1)
     assert np.all(np.abs(n - 1) < 1e-5)
@@ -192,9 +208,12 @@ def test_persistent_matches_plain(gl, mode):
     pipe.encode_persistent([b])
     torch.cuda.synchronize()
     ha, hb = a.to_host(), b.to_host()
-    assert np.mean(ha["termination"] == hb["termination"]) >= 0.9995
-    same = ha["termination"] == hb["termination"]
-    assert np.quantile(np.abs(ha["observable"][same] - hb["observable"][same]), 0.99) <= 1e-5
+    # every dispatch runs each ray on its own attempt clock (the same
+    # re-projection and drift schedule) through one step routine whose
+    # multiply-adds are fused per source expression only, so whole tiles and
+    # lane refill give the plain dispatch's outputs bit for bit
+    for k in ha:
+        assert np.array_equal(np.asarray(ha[k]), np.asarray(hb[k])), k
     stats = pipe.queue_stats()
     assert stats["items"] >= sc.rays
     assert 0 < stats["attempted_steps"] <= stats["lane_slots"]
@@ -245,9 +264,12 @@ def test_multi_scene_persistent_matches_single_scene_launches(gl):
     for m, s, t in zip(multi, single, tiled):
         hm, hs, ht = m.to_host(), s.to_host(), t.to_host()
         assert np.all(hm["termination"] > 0)
-        assert np.all(ht["termination"] > 0)
-        assert np.mean(hm["termination"] == hs["termination"]) >= 0.999
-        assert np.mean(ht["termination"] == hs["termination"]) >= 0.999
+        for k in hs:
+            # one lane-refill launch over many scenes equals one per scene
+            assert np.array_equal(np.asarray(hm[k]), np.asarray(hs[k])), k
+        for k in hs:
+            # and whole tiles trace each ray as lane refill does
+            assert np.array_equal(np.asarray(ht[k]), np.asarray(hs[k])), k
 
 
 def test_uneven_scenes_match_the_oracle(gl, oracle):
@@ -271,7 +293,9 @@ def test_uneven_scenes_match_the_oracle(gl, oracle):
         print(i, f"a={sc.spin:.3f} Q={sc.charge:.3f} inc={sc.inclination_deg:.1f} "
               f"side={sc.width}", np.bincount(A["termination"], minlength=8)[[1, 2, 7]],
               "flagged", int(A["flagged"].sum()), flush=True)
-        _compare(h, A, idx)
+        # a up to 0.998 with pixels on the critical curve: a few of the 256
+        # stay Uncertain (near-critical), none leaves with a wrong class
+        _compare(h, A, idx, dict(TOL, uncertain=1.2e-2))
 
 
 @pytest.mark.parametrize("charge,chart,radius,inclination", [
@@ -301,10 +325,16 @@ def test_static_observer_shadow_is_synges_disc(gl, charge, chart, radius, inclin
     inside, outside = rel < -2e-4, rel > 2e-4
     assert inside.sum() > 10000 and outside.sum() > 10000
     for h in (a.to_host(), b.to_host()):
-        assert np.all(h["termination"][inside] == 1)
-        assert np.all(h["termination"][outside] == 2)
+        # the class the trace reaches is Synge's; close to the edge a ray may
+        # stay Uncertain (near-critical), never a wrong definite class
+        assert np.all(h["class_reached"][inside] == 1)
+        assert np.all(h["class_reached"][outside] == 2)
+        assert np.all(np.isin(h["termination"][inside], (1, 6)))
+        assert np.all(np.isin(h["termination"][outside], (2, 6)))
+        far = np.abs(rel) > 2e-2
+        assert np.all(h["termination"][far] != 6)
         near = np.abs(rel) <= 2e-4
-        assert np.all(np.isin(h["termination"][near], (1, 2)))
+        assert np.all(np.isin(h["termination"][near], (1, 2, 6)))
 
 
 def test_geodesic_batch_reproduces_the_b6_fixture(gl):
@@ -341,7 +371,8 @@ def test_frame_graph_replays_match_eager(gl):
         graph.launch()
         graph.stream.synchronize()
         h = tgt.to_host()
-        assert np.mean(h["termination"] == he["termination"]) >= 0.9995
+        for k in he:
+            assert np.array_equal(np.asarray(h[k]), np.asarray(he[k])), k
         pipe.encode_persistent([eager])  # eager work between replays
         torch.cuda.synchronize()
 
@@ -421,10 +452,88 @@ def test_mixed_work_list_matches_separate_launches(gl, mode):
         torch.cuda.synchronize()
         hi, hr = images[i].to_host(), ref.to_host()
         assert np.all(hi["termination"] > 0)
-        assert np.mean(hi["termination"] == hr["termination"]) >= 0.999
+        for k in hr:
+            assert np.array_equal(np.asarray(hi[k]), np.asarray(hr[k])), (i, k)
         bt, states, delta = batches[i]
         hb = bt.to_host()
-        hp = gl.GeodesicBatch(pipe, i).trace_ordered(states, -np.ones(len(states)))
+        tb = gl.TraceTarget(len(states))
+        gl.GeodesicBatch(pipe, i).encode(torch.from_numpy(states).cuda(),
+                                         -torch.ones(len(states), dtype=torch.float32,
+                                                     device="cuda"), tb)
+        torch.cuda.synchronize()
+        hp = tb.to_host()
         assert np.all(hb["termination"] > 0)
-        big = np.abs(delta) >= 1e-4
-        assert np.array_equal(hb["termination"][big], hp["termination"][big])
+        for k in hp:
+            assert np.array_equal(np.asarray(hb[k]), np.asarray(hp[k])), (i, k)
+
+
+@pytest.mark.parametrize("refine", [False, True], ids=["f32", "refined"])
+def test_shadow_edge_requests_match_the_oracle(gl, oracle, refine):
+    """The uneven workload's shadow-edge rays (10% of each scene's pixel
+    count as geodesic-batch requests at offsets delta ~ +-log-U[1e-6, 1e-2]
+    from the critical curve) of four seed-drawn scenes, traced in one mixed
+    whole-tile launch with their images: on 192 seeded requests of each, every
+    definite class and event id equals the oracle's (rtol 1e-10, flagged when
+    the class changes at 1e-10 / 16) and every settled branch key is exact.
+    The f32 trace leaves the requests within its near-critical margin
+    Uncertain (a third of them, the offsets below ~1e-5); with the f64 refine
+    pass at most 1% stay Uncertain."""
+    from gravlume_amd import scene as S
+    import torch
+    scenes = S.uneven_scenes(count=4, seed=20261206)
+    for s in scenes:
+        s.width = s.height = 128
+        s.refine = refine
+    pipe = gl.TracePipeline(scenes)
+    items, keep = [], []
+    for i, s in enumerate(scenes):
+        items.append(("image", i, pipe.create_target(i)))
+        states, delta = S.shadow_edge_requests(s, 0.1, seed=700 + i)
+        d_states = torch.from_numpy(states).cuda()
+        d_dirs = -torch.ones(len(states), dtype=torch.float32, device="cuda")
+        bt = gl.TraceTarget(len(states))
+        items.append(("batch", i, d_states, d_dirs, bt))
+        keep.append((d_states, d_dirs, bt, states, delta))
+    pipe.encode_work(items)
+    torch.cuda.synchronize()
+    rng = np.random.default_rng(17)
+    for i, s in enumerate(scenes):
+        _, _, bt, states, delta = keep[i]
+        h = bt.to_host()
+        assert np.all(h["termination"] > 0)
+        idx = np.sort(rng.choice(len(states), min(192, len(states)), replace=False))
+        st = oracle.spacetime(1.0, s.spin, s.charge, float(s.chart))
+        ev = oracle.events(s.escape_radius, s.disk_radii(), horizon_offset=s.horizon_offset)
+        dirs = -np.ones(len(idx))
+        A = oracle.outcomes_to_arrays(oracle.trace_ordered(
+            st, oracle.config_policy(1e-10, 1e-12), ev, states[idx], dirs, threads=16))
+        F = oracle.outcomes_to_arrays(oracle.trace_ordered(
+            st, oracle.config_policy(1e-10 / 16, 1e-12 / 16), ev, states[idx], dirs, threads=16))
+        ok = (A["termination"] == F["termination"]) & (A["candidates"] == F["candidates"])
+        t = h["termination"][idx]
+        definite = ok & (t != 6)
+        print(i, f"a={s.spin:.3f} Q={s.charge:.3f}", "requests", len(states), "flagged",
+              int((~ok).sum()), "uncertain", int((ok & (t == 6)).sum()),
+              "branch unsettled", int((definite & h["branch_unsettled"][idx]).sum()))
+        assert np.mean(t[ok] == 6) <= (0.01 if refine else 0.5)
+        assert np.array_equal(t[definite], A["termination"][definite])
+        assert np.array_equal(h["candidates"][idx][definite], A["candidates"][definite])
+        settled = definite & ~h["branch_unsettled"][idx]
+        for key in ("radial_turnings", "equatorial_crossings", "azimuth_winding", "polar_side"):
+            assert np.array_equal(h[key][idx][settled], A[key][settled]), key
+
+
+def test_default_view_produces_only_determinate_rays(gl):
+    """The reference's default view at 80x45 (gpu_trace_tests.rs:356-378):
+    every ray crosses the horizon or escapes, and both occur."""
+    from gravlume_amd import scene as S
+    import torch
+    sc = S.Scene(spin=0.8, charge=0.0, inclination_deg=60.0, observer_radius=30.0, width=80,
+                 height=45, fov_deg=45.0, chart=S.OUTGOING, disk=None, escape_radius=200.0,
+                 escape_map=False)
+    pipe = gl.TracePipeline([sc])
+    tgt = pipe.create_target(0)
+    pipe.encode_batch(tgt)
+    torch.cuda.synchronize()
+    t = tgt.to_host()["termination"]
+    assert set(np.unique(t)) == {1, 2}, np.unique(t, return_counts=True)

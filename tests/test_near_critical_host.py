@@ -1,0 +1,134 @@
+"""The near-critical margin (gravlume_amd/csrc/near_critical.h) on the host.
+
+The margin mu = R(r_m) / (magnitude of R's two terms at r_m) of the radial
+potential R(r) = (E (r^2 + a^2) - a L)^2 - Delta ((L - a E)^2 + Q) at its
+minimum r_m over [r_+, r_start] decides capture (mu > 0) against scatter
+(mu < 0). Checked against what the mathematics fixes:
+
+* Schwarzschild: R = E^2 r^4 - b^2 E^2 (r^2 - 2 r), minimum at the photon
+  sphere r = 3 for b^2 = 27 where R vanishes; mu(b) = (27 - b^2)/(27 + b^2)
+  exactly at r_m = 3 once b^2 = 27, and sign(mu) = sign(27 - b^2) nearby;
+* Reissner-Nordstrom: the photon sphere (3 + sqrt(9 - 8 Q^2)) / 2 and the
+  critical impact parameter b^2 = r^4 / Delta(r) at it give mu = 0 there;
+* Kerr-Newman spherical photon orbits (gravlume_amd.critical, an independent
+  closed form of the double roots): mu vanishes on the critical curve, is
+  negative outside it (the rays escape) and positive inside (captured), and
+  is proportional to the offset delta for small delta, at a = 0.998 as well;
+* the f32 start error of a state whose momentum f32 holds exactly reduces
+  to that state's own null residual; for a generic f64 state it is binary32
+  rounding amplified by the observer radius in Carter's terms.
+"""
+import ctypes as C
+import math
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+
+
+@pytest.fixture(scope="module")
+def nc(tmp_path_factory):
+    if shutil.which("g++") is None:
+        pytest.skip("needs g++")
+    so = tmp_path_factory.mktemp("nc") / "libnc.so"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-o", str(so),
+                    os.path.join(HERE, "cpp", "near_critical_shim.cpp")], check=True)
+    lib = C.CDLL(str(so))
+    dp = C.POINTER(C.c_double)
+    lib.nc_state_margin.argtypes = [dp, C.c_size_t, C.c_double, C.c_double, C.c_double,
+                                    C.c_double, dp, dp]
+    lib.nc_radial_margin.argtypes = [C.c_double] * 7 + [dp]
+    lib.nc_f32_start_error.argtypes = [dp, C.c_double, C.c_double, C.c_double]
+    lib.nc_f32_start_error.restype = C.c_double
+    return lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def radial(nc, E, L, Q, a, q2, rp, r0):
+    out = np.zeros(3)
+    nc.nc_radial_margin(E, L, Q, a, q2, rp, r0, _dp(out))
+    return out
+
+
+def test_schwarzschild_photon_sphere(nc):
+    for b2 in (27.0 * (1 + 1e-6), 27.0 * (1 - 1e-6), 27.0 * 1.1, 27.0 * 0.9, 27.0):
+        mu, rm, ok = radial(nc, 1.0, math.sqrt(b2), 0.0, 0.0, 0.0, 2.0, 50.0)
+        assert ok
+        # R' = 4 r^3 - 2 b^2 r + 2 b^2: largest root, clamped to r_+ = 2
+        roots = np.roots([4.0, 0.0, -2.0 * b2, 2.0 * b2])
+        real = roots[np.abs(roots.imag) < 1e-9].real
+        r_expect = max(2.0, real.max()) if real.size == 3 else 2.0
+        assert rm == pytest.approx(r_expect, rel=1e-12)
+        R = rm ** 4 - b2 * (rm * rm - 2 * rm)
+        S = rm ** 4 + b2 * abs(rm * rm - 2 * rm)
+        assert mu == pytest.approx(R / S, rel=1e-10, abs=1e-15)
+        assert np.sign(mu) == np.sign(27.0 - b2) or b2 == 27.0
+    mu, rm, _ = radial(nc, 1.0, math.sqrt(27.0), 0.0, 0.0, 0.0, 2.0, 50.0)
+    assert rm == pytest.approx(3.0, rel=1e-12) and abs(mu) < 1e-14
+    # the same critical impact parameter split between L_z and Carter's Q
+    mu, rm, _ = radial(nc, 1.0, 3.0, 18.0, 0.0, 0.0, 2.0, 50.0)
+    assert rm == pytest.approx(3.0, rel=1e-12) and abs(mu) < 1e-14
+
+
+def test_reissner_nordstrom_photon_sphere(nc):
+    for q in (0.3, 0.7, 0.9):
+        rp = 1 + math.sqrt(1 - q * q)
+        rps = 0.5 * (3 + math.sqrt(9 - 8 * q * q))
+        b2 = rps ** 4 / (rps * rps - 2 * rps + q * q)
+        mu, rm, ok = radial(nc, 1.0, math.sqrt(b2), 0.0, 0.0, q * q, rp, 80.0)
+        assert ok and rm == pytest.approx(rps, rel=1e-10) and abs(mu) < 1e-13
+        assert radial(nc, 1.0, math.sqrt(b2 * (1 + 1e-5)), 0.0, 0.0, q * q, rp, 80.0)[0] < 0
+        assert radial(nc, 1.0, math.sqrt(b2 * (1 - 1e-5)), 0.0, 0.0, q * q, rp, 80.0)[0] > 0
+
+
+@pytest.mark.parametrize("a,q,r_obs,incl", [(0.998, 0.05, 200.0, 85.0), (0.94, 0.0, 100.0, 80.0),
+                                            (0.5, 0.3, 50.0, 60.0)])
+def test_kerr_newman_critical_curve(nc, a, q, r_obs, incl):
+    from gravlume_amd import critical as Cr
+    th = math.radians(incl)
+    states, delta, xi, eta = Cr.near_critical_states(a, q, r_obs, th, 400, seed=7, chart=-1)
+    rp = 1 + math.sqrt(1 - a * a - q * q)
+    mu = np.zeros(len(states))
+    rm = np.zeros(len(states))
+    assert nc.nc_state_margin(_dp(states), len(states), a, q * q, -1.0, rp, _dp(mu), _dp(rm)) == 0
+    # captured inside the curve (delta < 0), scattered outside
+    assert np.all(np.sign(mu) == -np.sign(delta))
+    ratio = np.abs(mu) / np.abs(delta)
+    assert ratio.min() > 1e-2 and ratio.max() < 1e4, (ratio.min(), ratio.max())
+    # the minimum sits at the spherical photon orbit of the curve point: the
+    # radii the curve spans
+    r1, r2 = Cr.circular_photon_orbits(a, q)
+    assert rm.min() >= r1 * (1 - 1e-3) and rm.max() <= r2 * (1 + 1e-3)
+    # on the curve itself (delta = 0 exactly, in constants): mu vanishes
+    r = np.linspace(r1, r2, 64)
+    xs, es = Cr.photon_orbit_constants(a, q, r)
+    for x_, e_, r_ in zip(xs, es, r):
+        if e_ < 0:
+            continue
+        m = radial(nc, 1.0, float(x_), float(e_), a, q * q, rp, r_obs)
+        assert abs(m[0]) < 1e-9, (r_, m)
+        assert m[1] == pytest.approx(r_, rel=1e-6, abs=1e-6)
+
+
+def test_f32_start_error(nc):
+    from gravlume_amd import critical as Cr
+    a, q = 0.94, 0.0
+    states, *_ = Cr.near_critical_states(a, q, 100.0, math.radians(80.0), 16, seed=3, chart=-1)
+    for s in states:
+        e = nc.nc_f32_start_error(_dp(np.ascontiguousarray(s)), a, q * q, -1.0)
+        # binary32 rounding of p times the lever arm r_obs of Carter's terms
+        assert 1e-10 < e < 1e-5, e
+        exact = s.copy()
+        exact[4:] = exact[4:].astype(np.float32).astype(np.float64)
+        e0 = nc.nc_f32_start_error(_dp(exact), a, q * q, -1.0)
+        # no constant changes; what remains is the rounded state's own null
+        # residual (binary32 rounding of the momentum)
+        assert e0 <= e and e0 < 5e-8, (e0, e)
