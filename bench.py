@@ -42,9 +42,12 @@ def parse():
     ap.add_argument("--no-accel", action="store_true", help="skip the accelerated sky leg")
     ap.add_argument("--no-footprint", action="store_true", help="skip the surface footprint leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--accuracy-rays", type=int, default=2048)
-    ap.add_argument("--persistent-config", type=str, default=None,
-                    help="WAVES,INTERVAL,THRESHOLD instead of autotuning")
+    ap.add_argument("--accuracy-rays", type=int, default=32768)
+    ap.add_argument("--persistent-config", type=str, default="2,8,64",
+                    help="WAVES,INTERVAL,THRESHOLD of the persistent launch (default: whole-tile "
+                         "mode, two waves per SIMD, the variant the committed profiles cover)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="pick the persistent launch configuration with the autotuner instead")
     ap.add_argument("--copy-calibration-mib", type=int, default=0)
     return ap.parse_args()
 
@@ -64,6 +67,14 @@ def measured_traffic(path=os.path.join(PROFILES, "k_persistent_tiles_1080p_traff
     except OSError:
         pass
     return None
+
+
+def kernel_variant(cfg: dict) -> str:
+    """Name of the persistent kernel instantiation a configuration launches
+    on a one-scene frame."""
+    if cfg["refill_threshold"] >= 64:
+        return f"k_persistent_tiles<true,{cfg['waves_per_simd']}>"
+    return f"k_persistent<true,{cfg['waves_per_simd']}>"
 
 
 def angle(a, b):
@@ -106,7 +117,9 @@ def cpu_baseline(scene, gpu_host, seconds, accuracy_rays, seed=20261203):
     elapsed = time.perf_counter() - t0
     base = {"value": done / elapsed, "unit": "rays/s", "cores": cores, "kind": "port",
             "sample": f"{done} seeded pixels of the 1920x1080 frame, f64 DP5 rtol {scene.rtol:g} "
-                      f"atol {scene.atol:g}, {elapsed:.1f} s on {cores} threads",
+                      f"atol {scene.atol:g} with the reference policy's 0.5 M step cap (the GPU "
+                      f"caps steps at 0.25 r, up to {scene.maximum_step:g} M), {elapsed:.1f} s on "
+                      f"{cores} threads",
             "host": {"nproc": os.cpu_count(), "cpu_model": _cpu_model()}}
     # the small CPU configuration in full: 64x64, a = 0.5, Q = 0.3, rtol 1e-10
     from gravlume_amd import scene as S
@@ -137,10 +150,16 @@ def cpu_baseline(scene, gpu_host, seconds, accuracy_rays, seed=20261203):
     gt = gpu_host["termination"][idx]
     gc = gpu_host["candidates"][idx]
     ok = ~flagged
+    definite = ok & (gt != 6)
     agree = float(np.mean((gt == A["termination"])[ok])) if ok.any() else float("nan")
     agree_ev = float(np.mean(((gt == A["termination"]) & (gc == A["candidates"]))[ok])) if ok.any() else float("nan")
     err = {"rays": int(accuracy_rays), "near_critical_flagged": int(flagged.sum()),
-           "termination_agreement": agree, "event_id_agreement": agree_ev}
+           "termination_agreement": agree, "event_id_agreement": agree_ev,
+           "uncertain": int(np.sum(ok & (gt == 6))),
+           "definite_class_mismatches": int(np.sum(definite & (gt != A["termination"]))),
+           "definite_event_id_mismatches": int(np.sum(definite & (gc != A["candidates"]))),
+           "event_residual_max": float(np.max(np.abs(gpu_host["event_residual"][idx][
+               definite & np.isin(gt, (1, 2, 3, 7))])))}
     esc = ok & (gt == 2) & (A["termination"] == 2)
     if esc.any():
         a = angle(gpu_host["observable"][idx][esc], A["escape_direction"][esc])
@@ -155,11 +174,12 @@ def cpu_baseline(scene, gpu_host, seconds, accuracy_rays, seed=20261203):
         err.update(g_rel_p99=float(np.quantile(rel, 0.99)), g_rel_max=float(rel.max()),
                    source_r_abs_max=float(np.max(np.abs(g[:, 0] - A["source_radius"][srf]))))
     err["null_drift_p99"] = float(np.quantile(gpu_host["drift"][idx][:, 0], 0.99))
-    # branch key on the rays whose class agrees with the oracle's
-    same = ok & (gt == A["termination"])
+    # branch key on the definite rays whose branch the trace settles
+    same = definite & ~gpu_host["branch_unsettled"][idx]
+    err["branch_settled_rays"] = int(same.sum())
     if same.any():
-        err["branch_key_agreement"] = {
-            k: float(np.mean((gpu_host[k][idx] == A[k])[same]))
+        err["branch_key_mismatches"] = {
+            k: int(np.sum((gpu_host[k][idx] != A[k])[same]))
             for k in ("radial_turnings", "equatorial_crossings", "azimuth_winding", "polar_side")}
     return base, err
 
@@ -216,26 +236,31 @@ def near_critical_leg(g, S, torch, n=1_000_000, sample=1024, seed=20261208):
     """Near-critical batch: a = 0.998, Q = 0.05, observer r = 200 M at 85 deg;
     n requests at offsets delta ~ +-log-U[1e-8, 1e-3] from the critical curve
     (f64 initial data), f32 DP5 rtol 1e-6 with the f64 steps inside
-    1.05 r_+; plain batch kernel against the persistent lane-refill queue;
-    class / winding / travel-time agreement with the f64 oracle at rtol
-    1e-12 on a seeded sample, by |delta| band."""
+    1.05 r_+, the interval capture certificate enabled. Three runs of the
+    same batch: the plain batch kernel and the persistent lane-refill queue
+    (rays the f32 trace cannot settle leave Uncertain), and the persistent
+    queue with the f64 refine pass at rtol 1e-12 (the oracle's tolerance).
+    Against the f64 oracle at rtol 1e-12 on a seeded sample, by |delta|
+    band: the Uncertain share, the class agreement of definite rays, and the
+    winding and travel time of definite rays whose branch is settled."""
     import math
     from gravlume_amd import critical as Cr
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
-    sc = S.near_critical_scene()
-    pipe = g.TracePipeline([sc])
-    pipe.enable_timing(True)
-    states, delta, _, _ = Cr.near_critical_states(sc.spin, sc.charge, sc.observer_radius,
-                                                  math.radians(sc.inclination_deg), n, seed=seed,
-                                                  chart=sc.chart)
+    states, delta, _, _ = Cr.near_critical_states(0.998, 0.05, 200.0, math.radians(85.0), n,
+                                                  seed=seed, chart=-1)
     d_states = torch.from_numpy(states).cuda()
     d_dirs = -torch.ones(n, dtype=torch.float32, device="cuda")
-    batch = g.GeodesicBatch(pipe, 0)
     res = {"workload": f"{n} rays, Kerr-Newman a=0.998 Q=0.05, observer r=200M incl 85deg, "
                        "signed offsets delta ~ log-U[1e-8,1e-3] from the critical curve, "
-                       "f32 DP5 rtol 1e-6 + f64 steps inside 1.05 r+"}
-    tgt = {}
-    for name, persistent in (("plain", False), ("persistent", True)):
+                       "f32 DP5 rtol 1e-6 + f64 steps inside 1.05 r+, capture certificate on; "
+                       "refined: near-critical rays traced again in f64 at rtol 1e-12"}
+    hosts = {}
+    for name, persistent, refine in (("plain", False, False), ("persistent", True, False),
+                                     ("persistent_refined", True, True)):
+        sc = S.near_critical_scene(capture_certificate=True, refine=refine)
+        pipe = g.TracePipeline([sc])
+        pipe.enable_timing(True)
+        batch = g.GeodesicBatch(pipe, 0)
         t = g.TraceTarget(n)
         batch.encode(d_states, d_dirs, t, persistent=persistent)  # warm-up
         torch.cuda.synchronize()
@@ -244,15 +269,28 @@ def near_critical_leg(g, S, torch, n=1_000_000, sample=1024, seed=20261208):
         batch.encode(d_states, d_dirs, t, persistent=persistent)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
-        main_ms, fin_ms, _ = pipe.kernel_ms()
+        main_ms, f64_ms, _ = pipe.kernel_ms()
+        h = t.to_host()
+        cert = (h["flags"] & 64) != 0
         res[name] = {"rays_per_s": n / wall, "ms": 1e3 * wall, "kernel_ms": main_ms,
-                     "f64_finisher_ms": fin_ms}
-        tgt[name] = t
+                     "f64_passes_ms": f64_ms, "uncertain": int(np.sum(h["termination"] == 6)),
+                     "near_critical": int(h["near_critical"].sum()),
+                     "branch_unsettled": int(h["branch_unsettled"].sum()),
+                     "refined": int(h["refined"].sum())}
+        res[name]["certificate"] = {"accepted": int(cert.sum()), "fallback": int(n - cert.sum()),
+                                    "note": "the certificate's proven envelope is |a| <= 0.9; at "
+                                            "a = 0.998 every ray falls back to the full trace"}
+        hosts[name] = h
+        del pipe, t
     res["persistent_over_plain"] = res["persistent"]["rays_per_s"] / res["plain"]["rays_per_s"]
-    h = tgt["persistent"].to_host()
+    h = hosts["persistent"]
     res["steps"] = {"accepted_mean": float(h["steps"].mean()), "accepted_max": int(h["steps"].max()),
                     "rejected_mean": float(h["rejected"].mean())}
+    hr = hosts["persistent_refined"]
+    res["steps_refined"] = {"accepted_mean": float(hr["steps"].mean()),
+                            "accepted_max": int(hr["steps"].max())}
     from oracle import oracle as O
+    sc = S.near_critical_scene()
     idx = np.random.default_rng(seed + 1).choice(n, sample, replace=False)
     cores = max(1, min(16, os.cpu_count() or 1))
     st = O.spacetime(1.0, sc.spin, sc.charge, float(sc.chart))
@@ -263,34 +301,34 @@ def near_critical_leg(g, S, torch, n=1_000_000, sample=1024, seed=20261208):
     fine = O.outcomes_to_arrays(O.trace_ordered(st, O.config_policy(1e-12 / 16, 1e-14 / 16), ev,
                                                 states[idx], dirs, threads=cores))
     t0 = time.perf_counter()
-    work = O.outcomes_to_arrays(O.trace_ordered(st, O.config_policy(sc.rtol, sc.atol), ev,
-                                                states[idx], dirs, threads=cores))
+    O.trace_ordered(st, O.config_policy(sc.rtol, sc.atol), ev, states[idx], dirs, threads=cores)
     t_work = time.perf_counter() - t0
     res["cpu_baseline_sample"] = {"rays_per_s": sample / t_work, "cores": cores, "kind": "port",
                                   "sample": f"{sample} seeded requests of the batch, f64 DP5 rtol "
-                                            f"{sc.rtol:g} atol {sc.atol:g}"}
+                                            f"{sc.rtol:g} atol {sc.atol:g}, 0.5 M step cap"}
     ok = ref["termination"] == fine["termination"]
-    settled = work["termination"] == ref["termination"]
-    bands = {}
     ad = np.abs(delta[idx])
-    for lo, hi in ((1e-8, 1e-6), (1e-6, 1e-5), (1e-5, 1e-3)):
-        m = ok & (ad >= lo) & (ad < hi)
-        same = h["termination"][idx][m] == ref["termination"][m]
-        f64_same = settled[m]
-        wind = h["azimuth_winding"][idx][m] == ref["azimuth_winding"][m]
-        f64_wind = work["azimuth_winding"][m] == ref["azimuth_winding"][m]
-        esc = m & (ref["termination"] == 2) & (h["termination"][idx] == 2)
-        tt = np.abs(h["travel_time"][idx][esc] / ref["travel_time"][esc] - 1.0)
-        tt64 = np.abs(work["travel_time"][esc] / ref["travel_time"][esc] - 1.0)
-        bands[f"{lo:g}-{hi:g}"] = {
-            "rays": int(m.sum()), "class_agreement": float(same.mean()) if m.any() else None,
-            "f64_at_rtol_1e-6_class_agreement": float(f64_same.mean()) if m.any() else None,
-            "winding_agreement": float(wind.mean()) if m.any() else None,
-            "f64_at_rtol_1e-6_winding_agreement": float(f64_wind.mean()) if m.any() else None,
-            "travel_time_rel_median": float(np.median(tt)) if esc.any() else None,
-            "f64_at_rtol_1e-6_travel_time_rel_median": float(np.median(tt64)) if esc.any() else None}
+    cmp = {}
+    for name in ("persistent", "persistent_refined"):
+        hh = hosts[name]
+        t = hh["termination"][idx]
+        bands = {}
+        for lo, hi in ((1e-8, 1e-6), (1e-6, 1e-5), (1e-5, 1e-3)):
+            m = ok & (ad >= lo) & (ad < hi)
+            d = m & (t != 6)
+            s_ = d & ~hh["branch_unsettled"][idx]
+            tt = np.abs(hh["travel_time"][idx][s_] / ref["travel_time"][s_] - 1.0)
+            bands[f"{lo:g}-{hi:g}"] = {
+                "rays": int(m.sum()), "uncertain_share": float(np.mean(t[m] == 6)) if m.any() else None,
+                "definite": int(d.sum()),
+                "class_agreement_definite": float(np.mean(t[d] == ref["termination"][d])) if d.any() else None,
+                "branch_settled": int(s_.sum()),
+                "winding_agreement_settled": float(np.mean(hh["azimuth_winding"][idx][s_] ==
+                                                           ref["azimuth_winding"][s_])) if s_.any() else None,
+                "travel_time_rel_max_settled": float(tt.max()) if s_.any() else None}
+        cmp[name] = bands
     res["vs_f64_rtol_1e-12"] = {"sample": sample, "oracle_flagged": int((~ok).sum()),
-                                "by_abs_delta": bands}
+                                "by_abs_delta": cmp}
     return res
 
 
@@ -495,7 +533,7 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    if args.persistent_config:
+    if not args.autotune:
         w, i, t = (int(v) for v in args.persistent_config.split(","))
         pipe.set_persistent_config(block_size=256, blocks_per_cu=w, refill_interval=i,
                                    refill_threshold=t, waves_per_simd=w)
@@ -575,7 +613,7 @@ def main():
         "config": {"workload": "single frame 1920x1080, Kerr a=0.94 Q=0, observer r=100M incl 80deg, "
                                "FOV 15deg, f32 DP5(4) rtol 1e-6 atol 1e-8, disk [r_ISCO, 30M], "
                                "escape r=1000M into a 2048x1024 seeded RGB map",
-                   "rays_per_step": rays, "kernel": "k_persistent"},
+                   "rays_per_step": rays, "kernel": kernel_variant(tuned["config"])},
         "plain_rays_per_s": plain_value,
         "host_inclusive_rays_per_s": rays * args.steps / t_pcie,
         "graph_rays_per_s": rays * args.steps / t_graph,
@@ -601,6 +639,9 @@ def main():
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "frac_of_nonpacked_fp32": achieved / 78.6,
                      "traffic": measured_traffic(),
+                     "traffic_from": "profiles/k_persistent_tiles_1080p_traffic.md: rocprofv3 --pmc "
+                                     "FETCH_SIZE and WRITE_SIZE of this kernel variant on this "
+                                     "frame, scaled by a 256 MiB copy (not measured in this run)",
                      "flops_per_launch": flops, "flops_per_attempt": attempt_flops,
                      "flops_per_rhs": rhs_flops, "attempts_per_launch": attempts},
         "cpu_baseline": None,
