@@ -1217,6 +1217,111 @@ gl_status gl_geodesic_batch_trace(gl_trace_plan* p, uint32_t scene, const double
   return launch_f64_passes(p, p->d_frame, df, st);
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free geodesic batch. The frame record, the four workspace
+// counters and the deferral and refine records of up to kNoAllocMaxRays
+// requests live in one file-scope device array, `scratch` (about 6.3 MB,
+// held on each GPU the library runs on until the process ends). Its device
+// address is taken once per device with hipGetSymbolAddress; host code uses
+// only that address. Every call on a GPU shares the array: calls that
+// overlap on one GPU race.
+namespace gl {
+
+constexpr uint32_t kNoAllocMaxRays = 32768u;
+constexpr size_t align256(size_t v) { return (v + 255u) & ~size_t(255u); }
+constexpr size_t kNoAllocCounters = align256(sizeof(FrameScene));
+constexpr size_t kNoAllocDefer = kNoAllocCounters + 256u;
+constexpr size_t kNoAllocRefine = align256(kNoAllocDefer + kNoAllocMaxRays * sizeof(DeferRecord));
+constexpr size_t kNoAllocBytes = align256(kNoAllocRefine + kNoAllocMaxRays * sizeof(RefineRecord));
+static_assert(alignof(FrameScene) <= 256 && alignof(DeferRecord) <= 256 &&
+                  alignof(RefineRecord) <= 256,
+              "scratch regions are 256-byte aligned");
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kNoAllocDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_address[kNoAllocDevices] = {};
+
+// The current device's address of `scratch`, looked up on the first call on
+// that device.
+gl_status scratch_address(void** out) {
+  int dev = -1;
+  GL_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNoAllocDevices)
+    return fail(GL_ERR_CAPACITY, "device ordinal beyond the allocation-free path's table");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_address[dev]) {
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || ptr == nullptr)
+      return fail(GL_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") +
+                                     hipGetErrorString(e));
+    g_scratch_address[dev] = ptr;
+  }
+  *out = g_scratch_address[dev];
+  return GL_OK;
+}
+
+// Writes the launch's frame record and clears the counters (a kernel, so a
+// captured call replays both).
+__global__ void k_noalloc_begin(FrameScene f, FrameScene* frame, unsigned int* counters) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    *frame = f;
+    counters[0] = 0;
+    counters[1] = 0;
+    counters[2] = 0;
+    counters[3] = 0;
+  }
+}
+
+}  // namespace gl
+
+gl_status gl_geodesic_batch_trace_noalloc(gl_trace_plan* p, uint32_t scene, const double* states8,
+                                          const float* directions, size_t n,
+                                          const gl_ray_outputs* out, void* stream) {
+  if (!p || !out) return fail(GL_ERR_INVALID_ARGUMENT, "plan and outputs are required");
+  if (scene >= p->scenes.size()) return fail(GL_ERR_INVALID_ARGUMENT, "scene index out of range");
+  if (n == 0) return GL_OK;
+  if (!states8 || !directions) return fail(GL_ERR_INVALID_ARGUMENT, "states and directions are required");
+  if (n > kNoAllocMaxRays)
+    return fail(GL_ERR_CAPACITY, "the allocation-free geodesic batch takes at most 32768 requests");
+  gl_status s = check_outputs(out, n);
+  if (s) return s;
+  void* base = nullptr;
+  s = scratch_address(&base);
+  if (s) return s;
+  unsigned char* b = static_cast<unsigned char*>(base);
+  FrameScene* frame = reinterpret_cast<FrameScene*>(b);
+  unsigned int* counters = reinterpret_cast<unsigned int*>(b + kNoAllocCounters);
+  const bool f64 = scene_f64_horizon(p->scenes[scene]), refine = scene_refine(p->scenes[scene]);
+  DeferBuf df;
+  df.rec = f64 ? reinterpret_cast<DeferRecord*>(b + kNoAllocDefer) : nullptr;
+  df.count = counters;
+  df.capacity = f64 ? kNoAllocMaxRays : 0u;
+  df.refine = refine ? reinterpret_cast<RefineRecord*>(b + kNoAllocRefine) : nullptr;
+  df.refine_capacity = refine ? kNoAllocMaxRays : 0u;
+  FrameScene f = make_frame(scene, 0, uint32_t(n), *out);
+  f.states = states8;
+  f.dirs = directions;
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_noalloc_begin, dim3(1), dim3(64), 0, st, f, frame, counters);
+  GL_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_geodesic_batch, dim3(uint32_t((n + 63) / 64)), dim3(64), 0, st,
+                     p->d_scenes, frame, states8, directions, uint32_t(n), map_view(p), df);
+  GL_HIP(hipGetLastError());
+  const dim3 grid(std::max(1, p->cu_count * 4));
+  if (df.rec) {
+    hipLaunchKernelGGL(k_finish_f64, grid, dim3(64), 0, st, p->d_scenes, frame, map_view(p), df);
+    GL_HIP(hipGetLastError());
+  }
+  if (df.refine) {
+    hipLaunchKernelGGL(k_refine_f64, grid, dim3(64), 0, st, p->d_scenes, frame, map_view(p), df);
+    GL_HIP(hipGetLastError());
+  }
+  return GL_OK;
+}
+
 gl_status gl_trace_plan_queue_stats(gl_trace_plan* p, gl_queue_stats* stats, void* stream) {
   if (!p || !stats) return fail(GL_ERR_INVALID_ARGUMENT, "plan and stats are required");
   QueueDev q;

@@ -374,8 +374,12 @@ class GeodesicBatch:
         self.scene = scene
         self.threads = threads
 
+    # requests per call of the allocation-free entry point
+    NO_ALLOC_MAX_REQUESTS = 32768
+
     def trace_ordered(self, states: np.ndarray, directions: np.ndarray, stream=None,
-                      persistent: bool = False, config: dict | None = None) -> dict:
+                      persistent: bool = False, config: dict | None = None,
+                      no_alloc: bool = False) -> dict:
         """Host arrays in, decoded host outputs back (synchronizes)."""
         states = np.ascontiguousarray(states, dtype=np.float64).reshape(-1, 8)
         n = states.shape[0]
@@ -388,18 +392,27 @@ class GeodesicBatch:
             return target.to_host()
         d_states = torch.from_numpy(states).cuda()
         d_dirs = torch.from_numpy(dirs).cuda()
-        self.encode(d_states, d_dirs, target, stream, persistent, config)
+        self.encode(d_states, d_dirs, target, stream, persistent, config, no_alloc)
         torch.cuda.synchronize()
         return target.to_host()
 
     def encode(self, d_states: torch.Tensor, d_dirs: torch.Tensor, target: TraceTarget,
-               stream=None, persistent: bool = False, config: dict | None = None) -> None:
+               stream=None, persistent: bool = False, config: dict | None = None,
+               no_alloc: bool = False) -> None:
         """Device arrays in (float64 (n, 8) states, float32 (n,) directions),
-        outputs into `target`; enqueued on `stream`, no synchronization."""
+        outputs into `target`; enqueued on `stream`, no synchronization.
+        ``no_alloc`` selects gl_geodesic_batch_trace_noalloc (no device
+        allocation; at most NO_ALLOC_MAX_REQUESTS requests per call)."""
         n = d_states.shape[0]
         out = target.c_struct()
         lib = self.pipeline._lib
-        if persistent:
+        if no_alloc:
+            if persistent:
+                raise ValueError("the allocation-free batch runs the plain batch kernel")
+            _capi.check(lib.gl_geodesic_batch_trace_noalloc(
+                self.pipeline.handle, self.scene, C.c_void_p(d_states.data_ptr()),
+                C.c_void_p(d_dirs.data_ptr()), n, C.byref(out), _stream_handle(stream)))
+        elif persistent:
             cfg = None
             if config is not None:
                 cfg = C.byref(_capi.PersistentConfig(

@@ -373,6 +373,19 @@ gl_status gl_geodesic_batch_trace_persistent(gl_trace_plan* plan, uint32_t scene
                                              size_t n, const gl_persistent_config* config,
                                              const gl_ray_outputs* out, void* stream);
 
+/* gl_geodesic_batch_trace without any device allocation, for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). No counterpart in the reference. The
+ * frame record, counters and f64 deferral/refine records live in a static
+ * device array of about 6.3 MB per GPU (held until the process ends); the
+ * caller passes the plan, inputs and outputs as for gl_geodesic_batch_trace,
+ * and gets the same results. At most 32768 requests per call (else
+ * GL_ERR_CAPACITY). Every call on one GPU shares that array: calls that
+ * overlap on one GPU (other streams, other threads) race. */
+gl_status gl_geodesic_batch_trace_noalloc(gl_trace_plan* plan, uint32_t scene,
+                                          const double* states8, const float* directions,
+                                          size_t n, const gl_ray_outputs* out, void* stream);
+
 /* Persistent-queue statistics of the last persistent launch. */
 typedef struct gl_queue_stats {
   uint64_t items;          /* rays handed out */
