@@ -46,13 +46,9 @@ constexpr float kStencilMaxError = 3.0e-4f;  // rad; half a pixel caps it furthe
 constexpr uint32_t kMaxNodeWait = 1u << 16;  // s_sleep rounds before a tile stops waiting
 
 __global__ void k_accel_reset(unsigned int* defer_count, uint32_t* edge_count, QueueDev* q) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    if (defer_count) {
-      defer_count[0] = 0;
-      defer_count[1] = 0;
-      defer_count[2] = 0;
-      defer_count[3] = 0;
-    }
+  if (blockIdx.x != 0) return;
+  if (defer_count) clear_counters(defer_count);
+  if (threadIdx.x == 0) {
     if (edge_count) edge_count[0] = 0;
     if (q) {
       q->next = 0;

@@ -27,8 +27,15 @@ struct DeferRecord {
 
 // Device workspace of one launch: the deferred near-horizon rays, the
 // near-critical rays for the f64 refine pass, and their counters
-// count[0] deferred records, [1] the finisher's claim cursor, [2] refine
-// records, [3] the refine pass's claim cursor.
+// (kCounterWords words): count[0] deferred records, [1] the finisher's claim
+// cursor, [2] refine records, [3] the refine pass's claim cursor,
+// [kDeferHist + b] deferred records in radius bucket b and
+// [kDeferFill + b] the ordering pass's cursor in bucket b.
+constexpr uint32_t kDeferBuckets = 32u;
+constexpr uint32_t kDeferHist = 32u;
+constexpr uint32_t kDeferFill = kDeferHist + kDeferBuckets;
+constexpr uint32_t kCounterWords = 128u;
+
 struct RefineRecord;
 struct DeferBuf {
   DeferRecord* rec;  // nullptr: no f64 near-horizon deferral in this launch
@@ -36,7 +43,26 @@ struct DeferBuf {
   uint32_t capacity;
   uint32_t refine_capacity;
   RefineRecord* refine;  // nullptr: no refine pass in this launch
+  uint32_t* order;       // the finisher's claim order over rec (capacity entries)
 };
+
+// The finisher's claim order groups deferred rays by the radius r_m of their
+// radial potential's minimum: the closer r_m lies to the horizon, the longer
+// the ray orbits inside the f64 island (on the a = 0.998 near-critical batch
+// rays with r_m < 1.09 take ~850 f64 attempts, rays with r_m > 2 two), so the
+// longest chains are claimed first. Bucket b covers log2 r_m in
+// [b / 16, (b + 1) / 16); r_m < 1 falls into bucket 0, r_m >= 4 and rays
+// without the test (r_m = 0) into the last.
+__device__ __forceinline__ uint32_t defer_bucket(float r_m) {
+  if (!(r_m > 0.0f)) return kDeferBuckets - 1u;
+  const float v = 16.0f * __log2f(fmaxf(r_m, 1.0f));
+  return v >= float(kDeferBuckets - 1u) ? kDeferBuckets - 1u : uint32_t(v);
+}
+
+// Clears the workspace counters; called by every lane of one wavefront.
+__device__ __forceinline__ void clear_counters(unsigned int* count) {
+  for (uint32_t w = threadIdx.x; w < kCounterWords; w += blockDim.x) count[w] = 0u;
+}
 
 // ---------------------------------------------------------------------------
 // Device helpers
@@ -583,6 +609,7 @@ __device__ __forceinline__ bool try_defer(const DeferBuf& df, const RayT<float>&
   df.rec[slot].ray = R;
   df.rec[slot].frame_scene = fs;
   df.rec[slot].index = idx;
+  atomicAdd(df.count + kDeferHist + defer_bucket(R.r_m), 1u);
   return true;
 }
 

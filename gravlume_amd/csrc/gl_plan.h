@@ -18,16 +18,17 @@ gl_status fail(gl_status st, const std::string& msg);
 
 namespace gl {
 // Device buffers of the f64 passes of a launch: the rays deferred to the
-// near-horizon finisher, the near-critical rays for the refine pass, and
-// their four counters (DeferBuf). The plan owns one for its eager calls;
+// near-horizon finisher with the finisher's claim order, the near-critical
+// rays for the refine pass, and their counters (DeferBuf). The plan owns one for its eager calls;
 // every frame graph owns its own, so a later call that needs more room can
 // reallocate the plan's without touching memory a graph replays into.
 struct Workspace {
   DeferRecord* defer = nullptr;
   uint32_t defer_capacity = 0;
+  uint32_t* order = nullptr;  // defer_capacity entries
   RefineRecord* refine = nullptr;
   uint32_t refine_capacity = 0;
-  unsigned int* counters = nullptr;  // 64 bytes, the first four used
+  unsigned int* counters = nullptr;  // kCounterWords words
 };
 }  // namespace gl
 

@@ -406,117 +406,235 @@ __device__ __forceinline__ RayT<float> narrow_ray(const RayT<double>& R) {
   return W;
 }
 
+// The f64 continuation of a deferred f32 ray: the position and the
+// coordinate time and affine parameter with their compensation folded in.
+__device__ __forceinline__ void widen_deferred(const RayT<float>& Rf, RayT<double>& R) {
+  R.x = double(Rf.x) - double(Rf.cx);
+  R.y = double(Rf.y) - double(Rf.cy);
+  R.z = double(Rf.z) - double(Rf.cz);
+  R.cx = R.cy = R.cz = 0.0;
+  R.px = Rf.px;
+  R.py = Rf.py;
+  R.pz = Rf.pz;
+  R.E = Rf.E;
+  R.h = Rf.h;
+  R.t = double(Rf.t) - double(Rf.ct);
+  R.ct = 0.0;
+  R.lam = double(Rf.lam) - double(Rf.clam);
+  R.clam = 0.0;
+  R.lz0 = Rf.lz0;
+  R.q0 = Rf.q0;
+  R.d_null = Rf.d_null;
+  R.d_lz = Rf.d_lz;
+  R.d_q = Rf.d_q;
+  R.accepted = Rf.accepted;
+  R.rejected = Rf.rejected;
+  R.consec = Rf.consec;
+  R.turnings = Rf.turnings;
+  R.crossings = Rf.crossings;
+  R.winding = Rf.winding;
+  R.armed = Rf.armed;
+  R.term = Rf.term;
+  R.candidates = Rf.candidates;
+  R.flags = Rf.flags;
+  R.polar_side = Rf.polar_side;
+  R.dir = Rf.dir;
+  R.r_turn = Rf.r_turn;
+  R.r_m = Rf.r_m;
+  R.d_crit = Rf.d_crit;
+}
+
+// The finisher's claim order (df.order): the deferred records by radius
+// bucket, bucket 0 first (defer_bucket: the longest f64 chains first); the
+// bucket counts were taken at deferral. Within a bucket the slots follow the
+// atomics; the order changes which lane finishes a ray and when, never its
+// outputs. When every lane of the finisher claims at most one record (as on
+// a 1080p frame's ~30,000 deferred rays) all chains start at once whatever
+// the order, and the deferral order, which keeps the rays of one tile
+// together in a wavefront, is kept.
+__global__ void __launch_bounds__(256) k_defer_order(DeferBuf df, uint32_t lanes) {
+  __shared__ uint32_t base[kDeferBuckets], local[kDeferBuckets], start[kDeferBuckets];
+  unsigned int n = df.count[0];
+  if (n > df.capacity) n = df.capacity;
+  if (n <= lanes) return;  // one claim per lane: the finisher keeps the deferral order
+  const uint32_t tid = threadIdx.x;
+  if (tid < kDeferBuckets) base[tid] = df.count[kDeferHist + tid];
+  __syncthreads();
+  if (tid == 0) {
+    uint32_t acc = 0;
+    for (uint32_t k = 0; k < kDeferBuckets; ++k) {
+      const uint32_t c = base[k];
+      base[k] = acc;
+      acc += c;
+    }
+  }
+  // 256 records per round: a bucket histogram in LDS, one global atomic per
+  // nonempty bucket to reserve its slots, then the scatter (the global
+  // cursors see one atomic per bucket and round, not one per record)
+  for (uint32_t first = blockIdx.x * 256u; first < n; first += gridDim.x * 256u) {
+    if (tid < kDeferBuckets) local[tid] = 0u;
+    __syncthreads();
+    const uint32_t i = first + tid;
+    const bool valid = i < n;
+    const uint32_t b = valid ? defer_bucket(df.rec[i].ray.r_m) : 0u;
+    const uint32_t rank = valid ? atomicAdd(&local[b], 1u) : 0u;
+    __syncthreads();
+    if (tid < kDeferBuckets && local[tid] != 0u)
+      start[tid] = base[tid] + atomicAdd(df.count + kDeferFill + tid, local[tid]);
+    __syncthreads();
+    if (valid) {
+      const uint32_t slot = start[b] + rank;
+      if (slot < n) df.order[slot] = i;
+    }
+    __syncthreads();
+  }
+}
+
+// The record behind claim i (< n): the deferral order when the finisher's
+// lanes take one record each, else the ordering pass's permutation of
+// [0, n) (an entry outside it falls back to record i).
+__device__ __forceinline__ uint32_t claimed_record(const DeferBuf& df, uint32_t i, uint32_t n) {
+  if (n <= gridDim.x * blockDim.x) return i;
+  const uint32_t r = df.order[i];
+  return r < n ? r : i;
+}
+
+// Rays that entered the f64 island finish with f64 DP5 steps. The lanes
+// refill one ray at a time from a claim cursor (df.count[1], through the
+// claim order) inside one loop: a lane whose ray ends writes it and takes
+// the next record at once, so a long f64 chain occupies its own lane only
+// and the other lanes of its wavefront keep stepping (f64 chains range from
+// one attempt to thousands).
 __global__ void __launch_bounds__(64) k_finish_f64(const SceneDev* __restrict__ scenes,
                                                    const FrameScene* __restrict__ frame,
                                                    MapView map, DeferBuf df) {
   unsigned int n = *df.count;
   if (n > df.capacity) n = df.capacity;
-  // Each lane takes the next deferred ray as soon as its last one finishes
-  // (df.count[1] is the claim cursor), so long f64 tails do not hold up the
-  // rays queued behind them on the same lane.
-  for (;;) {
-    const unsigned int i = atomicAdd(df.count + 1, 1u);
-    if (i >= n) break;
-    const DeferRecord& rec = df.rec[i];
-    const FrameScene& f = frame[rec.frame_scene];
-    const SceneDev& s = scenes[f.scene];
-    SceneT<double> sd = scene_f64(s);
-    if (f.out.turning_radius) sd.options |= kOptTurningRadius;
-    const RayT<float>& Rf = rec.ray;
-    RayT<double> R;
-    R.x = double(Rf.x) - double(Rf.cx);
-    R.y = double(Rf.y) - double(Rf.cy);
-    R.z = double(Rf.z) - double(Rf.cz);
-    R.cx = R.cy = R.cz = 0.0;
-    R.px = Rf.px;
-    R.py = Rf.py;
-    R.pz = Rf.pz;
-    R.E = Rf.E;
-    R.h = Rf.h;
-    R.t = double(Rf.t) - double(Rf.ct);
-    R.ct = 0.0;
-    R.lam = double(Rf.lam) - double(Rf.clam);
-    R.clam = 0.0;
-    R.lz0 = Rf.lz0;
-    R.q0 = Rf.q0;
-    R.d_null = Rf.d_null;
-    R.d_lz = Rf.d_lz;
-    R.d_q = Rf.d_q;
-    R.accepted = Rf.accepted;
-    R.rejected = Rf.rejected;
-    R.consec = Rf.consec;
-    R.turnings = Rf.turnings;
-    R.crossings = Rf.crossings;
-    R.winding = Rf.winding;
-    R.armed = Rf.armed;
-    R.term = Rf.term;
-    R.candidates = Rf.candidates;
-    R.flags = Rf.flags;
-    R.polar_side = Rf.polar_side;
-    R.dir = Rf.dir;
-    R.r_turn = Rf.r_turn;
-    R.r_m = Rf.r_m;
-    R.d_crit = Rf.d_crit;
+  RayT<double> R;
+  SceneT<double> sd;
+  double omega = 0.0;
+  uint32_t fs = 0, idx = 0, it = 1;
+  unsigned int i = atomicAdd(df.count + 1, 1u);
+  bool live = i < n;
+  if (live) {
+    const DeferRecord& rec = df.rec[claimed_record(df, i, n)];
+    fs = rec.frame_scene;
+    idx = rec.index;
+    const SceneDev& s = scenes[frame[fs].scene];
+    sd = scene_f64(s);
+    if (frame[fs].out.turning_radius) sd.options |= kOptTurningRadius;
+    omega = double(s.omega);
+    widen_deferred(rec.ray, R);
     if (!refresh_derivative<double>(sd, R)) {
       R.term = kTermFailure;
       R.flags |= kFlagNonFinite;
     }
-    for (uint32_t it = 1; R.term == kTermActive; ++it)
-      step<double>(sd, R, double(s.omega), false, drift_now(it));
-    write_ray(narrow_ray(R), f.out, rec.index, s, map, frame_source(f, s, rec.index),
-              refine_sink(df), rec.frame_scene);
   }
+  while (live) {
+    if (R.term == kTermActive) {
+      step<double>(sd, R, omega, false, drift_now(it));
+      ++it;
+    }
+    if (R.term != kTermActive) {
+      const FrameScene& f = frame[fs];
+      const SceneDev& s = scenes[f.scene];
+      write_ray(narrow_ray(R), f.out, idx, s, map, frame_source(f, s, idx), refine_sink(df), fs);
+      i = atomicAdd(df.count + 1, 1u);
+      live = i < n;
+      if (live) {
+        const DeferRecord& rec = df.rec[claimed_record(df, i, n)];
+        fs = rec.frame_scene;
+        idx = rec.index;
+        const SceneDev& s2 = scenes[frame[fs].scene];
+        sd = scene_f64(s2);
+        if (frame[fs].out.turning_radius) sd.options |= kOptTurningRadius;
+        omega = double(s2.omega);
+        widen_deferred(rec.ray, R);
+        it = 1;
+        if (!refresh_derivative<double>(sd, R)) {
+          R.term = kTermFailure;
+          R.flags |= kFlagNonFinite;
+        }
+      }
+    }
+  }
+}
+
+// The exact f64 start of a refine record's ray, with the scene's refine
+// tolerances.
+__device__ __forceinline__ void start_refined(const SceneDev* __restrict__ scenes,
+                                              const FrameScene* __restrict__ frame,
+                                              const RefineRecord& rec, SceneT<double>& sd,
+                                              RayT<double>& R, double& omega) {
+  const FrameScene& f = frame[rec.frame_scene];
+  const SceneDev& s = scenes[f.scene];
+  sd = scene_f64(s);
+  sd.rtol = double(s.refine_rtol);
+  sd.atol = double(s.refine_atol);
+  if (f.out.turning_radius) sd.options |= kOptTurningRadius;
+  omega = double(s.omega);
+  const RaySource src = frame_source(f, s, rec.index);
+  double y[8];
+  source_state(s, src.state8, src.px, src.py, y);
+  const float dir = f.states ? (f.dirs[rec.index] >= 0.0f ? 1.0f : -1.0f) : -1.0f;
+  const uint32_t side = f.states ? (y[3] > 0.0 ? 2u : (y[3] < 0.0 ? 0u : 1u)) : s.polar_side;
+  init_ray<double>(sd, R, y[1], y[2], y[3], 0.0, 0.0, 0.0, y[5], y[6], y[7], -y[4], dir, side);
+  R.h = double(dir) * double(s.h_init);
+  R.r_m = critical_radius(s, y);
 }
 
 // The f64 refine pass (GL_OPT_NEAR_CRITICAL_REFINE): every near-critical
 // ray of the launch is traced again from its exact f64 initial state, in
 // f64 throughout at the scene's refine tolerances, and written with the
 // near-critical test of that trace (an f64 start and f64 drifts). One ray
-// per lane from a claim cursor (df.count[3]), like the finisher.
+// per lane from a claim cursor (df.count[3]), refilled as in the finisher.
 __global__ void __launch_bounds__(64) k_refine_f64(const SceneDev* __restrict__ scenes,
                                                    const FrameScene* __restrict__ frame,
                                                    MapView map, DeferBuf df) {
   unsigned int n = df.count[2];
   if (n > df.refine_capacity) n = df.refine_capacity;
-  for (;;) {
-    const unsigned int i = atomicAdd(df.count + 3, 1u);
-    if (i >= n) break;
-    const RefineRecord rec = df.refine[i];
-    const FrameScene& f = frame[rec.frame_scene];
-    const SceneDev& s = scenes[f.scene];
-    SceneT<double> sd = scene_f64(s);
-    sd.rtol = double(s.refine_rtol);
-    sd.atol = double(s.refine_atol);
-    if (f.out.turning_radius) sd.options |= kOptTurningRadius;
-    const RaySource src = frame_source(f, s, rec.index);
-    double y[8];
-    source_state(s, src.state8, src.px, src.py, y);
-    const float dir = f.states ? (f.dirs[rec.index] >= 0.0f ? 1.0f : -1.0f) : -1.0f;
-    const uint32_t side = f.states ? (y[3] > 0.0 ? 2u : (y[3] < 0.0 ? 0u : 1u)) : s.polar_side;
-    RayT<double> R;
-    init_ray<double>(sd, R, y[1], y[2], y[3], 0.0, 0.0, 0.0, y[5], y[6], y[7], -y[4], dir, side);
-    R.h = double(dir) * double(s.h_init);
-    R.r_m = critical_radius(s, y);
-    for (uint32_t it = 1; R.term == kTermActive; ++it)
-      step<double>(sd, R, double(s.omega), false, drift_now(it));
-    write_ray(narrow_ray(R), f.out, rec.index, s, map, src, no_refine(), rec.frame_scene, true);
+  RayT<double> R;
+  SceneT<double> sd;
+  double omega = 0.0;
+  RefineRecord rec{0u, 0u};
+  uint32_t it = 1;
+  unsigned int i = atomicAdd(df.count + 3, 1u);
+  bool live = i < n;
+  if (live) {
+    rec = df.refine[i];
+    start_refined(scenes, frame, rec, sd, R, omega);
+  }
+  while (live) {
+    if (R.term == kTermActive) {
+      step<double>(sd, R, omega, false, drift_now(it));
+      ++it;
+    }
+    if (R.term != kTermActive) {
+      const FrameScene& f = frame[rec.frame_scene];
+      const SceneDev& s = scenes[f.scene];
+      write_ray(narrow_ray(R), f.out, rec.index, s, map, frame_source(f, s, rec.index),
+                no_refine(), rec.frame_scene, true);
+      i = atomicAdd(df.count + 3, 1u);
+      live = i < n;
+      if (live) {
+        rec = df.refine[i];
+        start_refined(scenes, frame, rec, sd, R, omega);
+        it = 1;
+      }
+    }
   }
 }
 
 __global__ void k_queue_reset(QueueDev* q, unsigned long long total, unsigned int* defer_count) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
+  if (blockIdx.x != 0) return;
+  if (defer_count) clear_counters(defer_count);  // deferral, refine and ordering counters
+  if (threadIdx.x == 0) {
     q->next = 0;
     q->total = total;
     q->refills = 0;
     q->attempts = 0;
     q->slots = 0;
     q->order_dirty = 0;
-    if (defer_count) {
-      defer_count[0] = 0;  // deferred rays
-      defer_count[1] = 0;  // finisher claim cursor
-      defer_count[2] = 0;  // refine records
-      defer_count[3] = 0;  // refine claim cursor
-    }
   }
 }
 
@@ -628,14 +746,17 @@ gl_status ensure_workspace(Workspace& w, size_t rays, bool f64, bool refine) {
   // written as near-critical and Uncertain (refine).
   const size_t want = std::min<size_t>(std::max<size_t>(rays, 1), size_t(1) << 22);
   if (!w.counters) {
-    GL_HIP(hipMalloc(&w.counters, 64));
-    GL_HIP(hipMemset(w.counters, 0, 64));
+    GL_HIP(hipMalloc(&w.counters, kCounterWords * sizeof(unsigned int)));
+    GL_HIP(hipMemset(w.counters, 0, kCounterWords * sizeof(unsigned int)));
   }
-  if (f64 && (want > w.defer_capacity || !w.defer)) {
+  if (f64 && (want > w.defer_capacity || !w.defer || !w.order)) {
     if (w.defer) (void)hipFree(w.defer);
+    if (w.order) (void)hipFree(w.order);
     w.defer = nullptr;
+    w.order = nullptr;
     w.defer_capacity = 0;
     GL_HIP(hipMalloc(&w.defer, want * sizeof(DeferRecord)));
+    GL_HIP(hipMalloc(&w.order, want * sizeof(uint32_t)));
     w.defer_capacity = uint32_t(want);
   }
   if (refine && (want > w.refine_capacity || !w.refine)) {
@@ -650,6 +771,7 @@ gl_status ensure_workspace(Workspace& w, size_t rays, bool f64, bool refine) {
 
 void free_workspace(Workspace& w) {
   if (w.defer) (void)hipFree(w.defer);
+  if (w.order) (void)hipFree(w.order);
   if (w.refine) (void)hipFree(w.refine);
   if (w.counters) (void)hipFree(w.counters);
   w = Workspace();
@@ -658,6 +780,7 @@ void free_workspace(Workspace& w) {
 DeferBuf workspace_view(const Workspace& w, bool f64, bool refine) {
   DeferBuf d;
   d.rec = f64 ? w.defer : nullptr;
+  d.order = f64 ? w.order : nullptr;
   d.count = w.counters;
   d.capacity = f64 ? w.defer_capacity : 0u;
   d.refine = refine ? w.refine : nullptr;
@@ -764,6 +887,9 @@ gl_status launch_f64_passes(gl_trace_plan* p, const FrameScene* table, const Def
   mark(p, 2, st);
   const dim3 grid(std::max(1, p->cu_count * 4));
   if (df.rec) {
+    hipLaunchKernelGGL(k_defer_order, dim3(std::max(1, p->cu_count)), dim3(256), 0, st, df,
+                       uint32_t(grid.x * 64u));
+    GL_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_finish_f64, grid, dim3(64), 0, st, p->d_scenes, table, map_view(p), df);
     GL_HIP(hipGetLastError());
   }
@@ -1053,8 +1179,10 @@ gl_status gl_trace_plan_create(const gl_scene_desc* scenes, size_t n_scenes,
     return cleanup(e, "upload scenes");
   if ((e = hipMalloc(&p->d_queue, sizeof(QueueDev))) != hipSuccess) return cleanup(e, "hipMalloc queue");
   if ((e = hipMemset(p->d_queue, 0, sizeof(QueueDev))) != hipSuccess) return cleanup(e, "clear queue");
-  if ((e = hipMalloc(&p->ws.counters, 64)) != hipSuccess) return cleanup(e, "hipMalloc counters");
-  if ((e = hipMemset(p->ws.counters, 0, 64)) != hipSuccess) return cleanup(e, "clear counters");
+  if ((e = hipMalloc(&p->ws.counters, kCounterWords * sizeof(unsigned int))) != hipSuccess)
+    return cleanup(e, "hipMalloc counters");
+  if ((e = hipMemset(p->ws.counters, 0, kCounterWords * sizeof(unsigned int))) != hipSuccess)
+    return cleanup(e, "clear counters");
   if (escape_map_rgb) {
     if (map_width == 0 || map_height == 0) {
       gl_trace_plan_destroy(p);
@@ -1230,8 +1358,9 @@ namespace gl {
 constexpr uint32_t kNoAllocMaxRays = 32768u;
 constexpr size_t align256(size_t v) { return (v + 255u) & ~size_t(255u); }
 constexpr size_t kNoAllocCounters = align256(sizeof(FrameScene));
-constexpr size_t kNoAllocDefer = kNoAllocCounters + 256u;
-constexpr size_t kNoAllocRefine = align256(kNoAllocDefer + kNoAllocMaxRays * sizeof(DeferRecord));
+constexpr size_t kNoAllocDefer = kNoAllocCounters + align256(kCounterWords * sizeof(unsigned int));
+constexpr size_t kNoAllocOrder = align256(kNoAllocDefer + kNoAllocMaxRays * sizeof(DeferRecord));
+constexpr size_t kNoAllocRefine = align256(kNoAllocOrder + kNoAllocMaxRays * sizeof(uint32_t));
 constexpr size_t kNoAllocBytes = align256(kNoAllocRefine + kNoAllocMaxRays * sizeof(RefineRecord));
 static_assert(alignof(FrameScene) <= 256 && alignof(DeferRecord) <= 256 &&
                   alignof(RefineRecord) <= 256,
@@ -1266,13 +1395,9 @@ gl_status scratch_address(void** out) {
 // Writes the launch's frame record and clears the counters (a kernel, so a
 // captured call replays both).
 __global__ void k_noalloc_begin(FrameScene f, FrameScene* frame, unsigned int* counters) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    *frame = f;
-    counters[0] = 0;
-    counters[1] = 0;
-    counters[2] = 0;
-    counters[3] = 0;
-  }
+  if (blockIdx.x != 0) return;
+  clear_counters(counters);
+  if (threadIdx.x == 0) *frame = f;
 }
 
 }  // namespace gl
@@ -1297,6 +1422,7 @@ gl_status gl_geodesic_batch_trace_noalloc(gl_trace_plan* p, uint32_t scene, cons
   const bool f64 = scene_f64_horizon(p->scenes[scene]), refine = scene_refine(p->scenes[scene]);
   DeferBuf df;
   df.rec = f64 ? reinterpret_cast<DeferRecord*>(b + kNoAllocDefer) : nullptr;
+  df.order = f64 ? reinterpret_cast<uint32_t*>(b + kNoAllocOrder) : nullptr;
   df.count = counters;
   df.capacity = f64 ? kNoAllocMaxRays : 0u;
   df.refine = refine ? reinterpret_cast<RefineRecord*>(b + kNoAllocRefine) : nullptr;
@@ -1312,6 +1438,9 @@ gl_status gl_geodesic_batch_trace_noalloc(gl_trace_plan* p, uint32_t scene, cons
   GL_HIP(hipGetLastError());
   const dim3 grid(std::max(1, p->cu_count * 4));
   if (df.rec) {
+    hipLaunchKernelGGL(k_defer_order, dim3(std::max(1, p->cu_count)), dim3(256), 0, st, df,
+                       uint32_t(grid.x * 64u));
+    GL_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_finish_f64, grid, dim3(64), 0, st, p->d_scenes, frame, map_view(p), df);
     GL_HIP(hipGetLastError());
   }
