@@ -23,18 +23,21 @@ struct DeferRecord {
   RayT<float> ray;
   uint32_t frame_scene;
   uint32_t index;
+  uint32_t bucket;  // the finisher's claim-order bucket (defer_bucket)
 };
 
 // Device workspace of one launch: the deferred near-horizon rays, the
-// near-critical rays for the f64 refine pass, and their counters
-// (kCounterWords words): count[0] deferred records, [1] the finisher's claim
-// cursor, [2] refine records, [3] the refine pass's claim cursor,
-// [kDeferHist + b] deferred records in radius bucket b and
-// [kDeferFill + b] the ordering pass's cursor in bucket b.
-constexpr uint32_t kDeferBuckets = 32u;
+// near-critical rays for the f64 refine pass, their claim orders and their
+// counters (kCounterWords words): count[0] deferred records, [1] the
+// finisher's claim cursor, [2] refine records, [3] the refine pass's claim
+// cursor, [kDeferHist + b] / [kRefineHist + b] records in claim bucket b,
+// [kDeferFill + b] / [kRefineFill + b] the ordering passes' cursors.
+constexpr uint32_t kClaimBuckets = 32u;
 constexpr uint32_t kDeferHist = 32u;
-constexpr uint32_t kDeferFill = kDeferHist + kDeferBuckets;
-constexpr uint32_t kCounterWords = 128u;
+constexpr uint32_t kDeferFill = kDeferHist + kClaimBuckets;
+constexpr uint32_t kRefineHist = kDeferFill + kClaimBuckets;
+constexpr uint32_t kRefineFill = kRefineHist + kClaimBuckets;
+constexpr uint32_t kCounterWords = kRefineFill + kClaimBuckets + 32u;
 
 struct RefineRecord;
 struct DeferBuf {
@@ -42,21 +45,32 @@ struct DeferBuf {
   unsigned int* count;
   uint32_t capacity;
   uint32_t refine_capacity;
-  RefineRecord* refine;  // nullptr: no refine pass in this launch
-  uint32_t* order;       // the finisher's claim order over rec (capacity entries)
+  RefineRecord* refine;    // nullptr: no refine pass in this launch
+  uint32_t* order;         // the finisher's claim order over rec (capacity entries)
+  uint32_t* refine_order;  // the refine pass's claim order (refine_capacity entries)
 };
 
-// The finisher's claim order groups deferred rays by the radius r_m of their
-// radial potential's minimum: the closer r_m lies to the horizon, the longer
-// the ray orbits inside the f64 island (on the a = 0.998 near-critical batch
-// rays with r_m < 1.09 take ~850 f64 attempts, rays with r_m > 2 two), so the
-// longest chains are claimed first. Bucket b covers log2 r_m in
-// [b / 16, (b + 1) / 16); r_m < 1 falls into bucket 0, r_m >= 4 and rays
-// without the test (r_m = 0) into the last.
+// The f64 passes claim their longest chains first (longest-processing-time
+// order), by bucket, bucket 0 first. The finisher groups deferred rays by
+// the radius r_m of their radial potential's minimum: the closer r_m lies to
+// the horizon, the longer the ray orbits inside the f64 island (on the
+// a = 0.998 near-critical batch rays with r_m < 1.09 take ~850 f64 attempts,
+// rays with r_m > 2 two). Bucket b covers log2 r_m in [b / 16, (b + 1) / 16);
+// r_m < 1 falls into bucket 0, r_m >= 4 and rays without the test (r_m = 0)
+// into the last.
 __device__ __forceinline__ uint32_t defer_bucket(float r_m) {
-  if (!(r_m > 0.0f)) return kDeferBuckets - 1u;
+  if (!(r_m > 0.0f)) return kClaimBuckets - 1u;
   const float v = 16.0f * __log2f(fmaxf(r_m, 1.0f));
-  return v >= float(kDeferBuckets - 1u) ? kDeferBuckets - 1u : uint32_t(v);
+  return v >= float(kClaimBuckets - 1u) ? kClaimBuckets - 1u : uint32_t(v);
+}
+
+// The refine pass groups rays by the attempts their f32 trace took (its f64
+// trace at rtol 1e-12 takes ~16 times as many, correlated at 0.98 in log on
+// the near-critical batch): bucket 31 - floor(2 log2(attempts + 1)), so
+// bucket 0 holds rays of 2^15.5 attempts and more.
+__device__ __forceinline__ uint32_t refine_bucket(uint32_t attempts) {
+  const float v = 2.0f * __log2f(float(attempts) + 1.0f);
+  return v >= float(kClaimBuckets - 1u) ? 0u : kClaimBuckets - 1u - uint32_t(v);
 }
 
 // Clears the workspace counters; called by every lane of one wavefront.
@@ -365,16 +379,20 @@ __device__ __forceinline__ RaySource frame_source(const FrameScene& f, const Sce
 // Near-critical rays handed to the f64 refine pass: (frame slice, element).
 struct RefineRecord {
   uint32_t frame_scene, index;
+  uint32_t bucket;  // the refine pass's claim-order bucket (refine_bucket)
 };
 struct RefineSink {
   RefineRecord* rec;  // nullptr: no refine pass in this launch
   unsigned int* count;
   uint32_t capacity;
+  unsigned int* hist;  // records per claim bucket
 };
 __device__ __forceinline__ RefineSink refine_sink(const DeferBuf& df) {
-  return RefineSink{df.refine, df.count + 2, df.refine_capacity};
+  return RefineSink{df.refine, df.count + 2, df.refine_capacity, df.count + kRefineHist};
 }
-__device__ __forceinline__ RefineSink no_refine() { return RefineSink{nullptr, nullptr, 0u}; }
+__device__ __forceinline__ RefineSink no_refine() {
+  return RefineSink{nullptr, nullptr, 0u, nullptr};
+}
 
 __device__ __forceinline__ void source_state(const SceneDev& s, const double* state8, uint32_t px,
                                              uint32_t py, double y[8]) {
@@ -547,7 +565,9 @@ __device__ __forceinline__ bool write_ray(const RayT<float>& R, const OutPlanes&
       (s.options & kOptRefine) && sink.rec) {
     const unsigned int slot = atomicAdd(sink.count, 1u);
     if (slot < sink.capacity) {
-      sink.rec[slot] = RefineRecord{frame_scene, idx};
+      const uint32_t b = refine_bucket(R.accepted + R.rejected);
+      sink.rec[slot] = RefineRecord{frame_scene, idx, b};
+      atomicAdd(sink.hist + b, 1u);
       return false;
     }
   }
@@ -609,7 +629,9 @@ __device__ __forceinline__ bool try_defer(const DeferBuf& df, const RayT<float>&
   df.rec[slot].ray = R;
   df.rec[slot].frame_scene = fs;
   df.rec[slot].index = idx;
-  atomicAdd(df.count + kDeferHist + defer_bucket(R.r_m), 1u);
+  const uint32_t b = defer_bucket(R.r_m);
+  df.rec[slot].bucket = b;
+  atomicAdd(df.count + kDeferHist + b, 1u);
   return true;
 }
 

@@ -28,6 +28,7 @@ struct Workspace {
   uint32_t* order = nullptr;  // defer_capacity entries
   RefineRecord* refine = nullptr;
   uint32_t refine_capacity = 0;
+  uint32_t* refine_order = nullptr;  // refine_capacity entries
   unsigned int* counters = nullptr;  // kCounterWords words
 };
 }  // namespace gl

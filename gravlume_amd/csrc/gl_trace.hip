@@ -444,58 +444,68 @@ __device__ __forceinline__ void widen_deferred(const RayT<float>& Rf, RayT<doubl
   R.d_crit = Rf.d_crit;
 }
 
-// The finisher's claim order (df.order): the deferred records by radius
-// bucket, bucket 0 first (defer_bucket: the longest f64 chains first); the
-// bucket counts were taken at deferral. Within a bucket the slots follow the
-// atomics; the order changes which lane finishes a ray and when, never its
-// outputs. When every lane of the finisher claims at most one record (as on
-// a 1080p frame's ~30,000 deferred rays) all chains start at once whatever
-// the order, and the deferral order, which keeps the rays of one tile
-// together in a wavefront, is kept.
-__global__ void __launch_bounds__(256) k_defer_order(DeferBuf df, uint32_t lanes) {
-  __shared__ uint32_t base[kDeferBuckets], local[kDeferBuckets], start[kDeferBuckets];
-  unsigned int n = df.count[0];
-  if (n > df.capacity) n = df.capacity;
-  if (n <= lanes) return;  // one claim per lane: the finisher keeps the deferral order
+// The claim order of an f64 pass: its records by claim bucket, bucket 0
+// first (defer_bucket, refine_bucket: the longest chains first), from the
+// bucket counts taken when the records were written. Within a bucket the
+// slots follow the atomics; the order changes which lane traces a ray and
+// when, never its outputs. When every lane of the pass claims at most one
+// record (as on a 1080p frame's ~30,000 deferred rays) all chains start at
+// once whatever the order, and the write order, which keeps the rays of one
+// tile together in a wavefront, is kept. Records are `stride` bytes with
+// their bucket at `bucket_offset`; 256 records per round: a bucket
+// histogram in LDS, one global atomic per nonempty bucket to reserve its
+// slots, then the scatter.
+__global__ void __launch_bounds__(256) k_claim_order(const unsigned char* __restrict__ rec,
+                                                     uint32_t stride, uint32_t bucket_offset,
+                                                     const unsigned int* __restrict__ count_n,
+                                                     uint32_t capacity,
+                                                     const unsigned int* __restrict__ hist,
+                                                     unsigned int* fill, uint32_t* order,
+                                                     uint32_t lanes) {
+  __shared__ uint32_t base[kClaimBuckets], local[kClaimBuckets], start[kClaimBuckets];
+  unsigned int n = *count_n;
+  if (n > capacity) n = capacity;
+  if (n <= lanes) return;
   const uint32_t tid = threadIdx.x;
-  if (tid < kDeferBuckets) base[tid] = df.count[kDeferHist + tid];
+  if (tid < kClaimBuckets) base[tid] = hist[tid];
   __syncthreads();
   if (tid == 0) {
     uint32_t acc = 0;
-    for (uint32_t k = 0; k < kDeferBuckets; ++k) {
+    for (uint32_t k = 0; k < kClaimBuckets; ++k) {
       const uint32_t c = base[k];
       base[k] = acc;
       acc += c;
     }
   }
-  // 256 records per round: a bucket histogram in LDS, one global atomic per
-  // nonempty bucket to reserve its slots, then the scatter (the global
-  // cursors see one atomic per bucket and round, not one per record)
   for (uint32_t first = blockIdx.x * 256u; first < n; first += gridDim.x * 256u) {
-    if (tid < kDeferBuckets) local[tid] = 0u;
+    if (tid < kClaimBuckets) local[tid] = 0u;
     __syncthreads();
     const uint32_t i = first + tid;
     const bool valid = i < n;
-    const uint32_t b = valid ? defer_bucket(df.rec[i].ray.r_m) : 0u;
+    uint32_t b = 0u;
+    if (valid) {
+      b = *reinterpret_cast<const uint32_t*>(rec + size_t(i) * stride + bucket_offset);
+      if (b >= kClaimBuckets) b = kClaimBuckets - 1u;
+    }
     const uint32_t rank = valid ? atomicAdd(&local[b], 1u) : 0u;
     __syncthreads();
-    if (tid < kDeferBuckets && local[tid] != 0u)
-      start[tid] = base[tid] + atomicAdd(df.count + kDeferFill + tid, local[tid]);
+    if (tid < kClaimBuckets && local[tid] != 0u)
+      start[tid] = base[tid] + atomicAdd(fill + tid, local[tid]);
     __syncthreads();
     if (valid) {
       const uint32_t slot = start[b] + rank;
-      if (slot < n) df.order[slot] = i;
+      if (slot < n) order[slot] = i;
     }
     __syncthreads();
   }
 }
 
-// The record behind claim i (< n): the deferral order when the finisher's
-// lanes take one record each, else the ordering pass's permutation of
-// [0, n) (an entry outside it falls back to record i).
-__device__ __forceinline__ uint32_t claimed_record(const DeferBuf& df, uint32_t i, uint32_t n) {
+// The record behind claim i (< n): the write order when the pass's lanes
+// take one record each, else the ordering pass's permutation of [0, n) (an
+// entry outside it falls back to record i).
+__device__ __forceinline__ uint32_t claimed_record(const uint32_t* order, uint32_t i, uint32_t n) {
   if (n <= gridDim.x * blockDim.x) return i;
-  const uint32_t r = df.order[i];
+  const uint32_t r = order[i];
   return r < n ? r : i;
 }
 
@@ -517,7 +527,7 @@ __global__ void __launch_bounds__(64) k_finish_f64(const SceneDev* __restrict__ 
   unsigned int i = atomicAdd(df.count + 1, 1u);
   bool live = i < n;
   if (live) {
-    const DeferRecord& rec = df.rec[claimed_record(df, i, n)];
+    const DeferRecord& rec = df.rec[claimed_record(df.order, i, n)];
     fs = rec.frame_scene;
     idx = rec.index;
     const SceneDev& s = scenes[frame[fs].scene];
@@ -542,7 +552,7 @@ __global__ void __launch_bounds__(64) k_finish_f64(const SceneDev* __restrict__ 
       i = atomicAdd(df.count + 1, 1u);
       live = i < n;
       if (live) {
-        const DeferRecord& rec = df.rec[claimed_record(df, i, n)];
+        const DeferRecord& rec = df.rec[claimed_record(df.order, i, n)];
         fs = rec.frame_scene;
         idx = rec.index;
         const SceneDev& s2 = scenes[frame[fs].scene];
@@ -596,12 +606,12 @@ __global__ void __launch_bounds__(64) k_refine_f64(const SceneDev* __restrict__ 
   RayT<double> R;
   SceneT<double> sd;
   double omega = 0.0;
-  RefineRecord rec{0u, 0u};
+  RefineRecord rec{0u, 0u, 0u};
   uint32_t it = 1;
   unsigned int i = atomicAdd(df.count + 3, 1u);
   bool live = i < n;
   if (live) {
-    rec = df.refine[i];
+    rec = df.refine[claimed_record(df.refine_order, i, n)];
     start_refined(scenes, frame, rec, sd, R, omega);
   }
   while (live) {
@@ -617,7 +627,7 @@ __global__ void __launch_bounds__(64) k_refine_f64(const SceneDev* __restrict__ 
       i = atomicAdd(df.count + 3, 1u);
       live = i < n;
       if (live) {
-        rec = df.refine[i];
+        rec = df.refine[claimed_record(df.refine_order, i, n)];
         start_refined(scenes, frame, rec, sd, R, omega);
         it = 1;
       }
@@ -759,11 +769,14 @@ gl_status ensure_workspace(Workspace& w, size_t rays, bool f64, bool refine) {
     GL_HIP(hipMalloc(&w.order, want * sizeof(uint32_t)));
     w.defer_capacity = uint32_t(want);
   }
-  if (refine && (want > w.refine_capacity || !w.refine)) {
+  if (refine && (want > w.refine_capacity || !w.refine || !w.refine_order)) {
     if (w.refine) (void)hipFree(w.refine);
+    if (w.refine_order) (void)hipFree(w.refine_order);
     w.refine = nullptr;
+    w.refine_order = nullptr;
     w.refine_capacity = 0;
     GL_HIP(hipMalloc(&w.refine, want * sizeof(RefineRecord)));
+    GL_HIP(hipMalloc(&w.refine_order, want * sizeof(uint32_t)));
     w.refine_capacity = uint32_t(want);
   }
   return GL_OK;
@@ -773,6 +786,7 @@ void free_workspace(Workspace& w) {
   if (w.defer) (void)hipFree(w.defer);
   if (w.order) (void)hipFree(w.order);
   if (w.refine) (void)hipFree(w.refine);
+  if (w.refine_order) (void)hipFree(w.refine_order);
   if (w.counters) (void)hipFree(w.counters);
   w = Workspace();
 }
@@ -784,6 +798,7 @@ DeferBuf workspace_view(const Workspace& w, bool f64, bool refine) {
   d.count = w.counters;
   d.capacity = f64 ? w.defer_capacity : 0u;
   d.refine = refine ? w.refine : nullptr;
+  d.refine_order = refine ? w.refine_order : nullptr;
   d.refine_capacity = refine ? w.refine_capacity : 0u;
   return d;
 }
@@ -878,6 +893,35 @@ const std::vector<float>* blackbody_table() {
   return table.empty() ? nullptr : &table;
 }
 
+// The claim orders and the f64 passes of a launch on `st`: the finisher
+// over however many rays were deferred, then the refine pass over however
+// many were near-critical (all counts read on the device).
+gl_status launch_f64_kernels(const gl_trace_plan* p, const FrameScene* table, const DeferBuf& df,
+                             hipStream_t st) {
+  const dim3 grid(std::max(1, p->cu_count * 4)), order_grid(std::max(1, p->cu_count));
+  const uint32_t lanes = grid.x * 64u;
+  if (df.rec) {
+    hipLaunchKernelGGL(k_claim_order, order_grid, dim3(256), 0, st,
+                       reinterpret_cast<const unsigned char*>(df.rec), uint32_t(sizeof(DeferRecord)),
+                       uint32_t(offsetof(DeferRecord, bucket)), df.count, df.capacity,
+                       df.count + kDeferHist, df.count + kDeferFill, df.order, lanes);
+    GL_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_finish_f64, grid, dim3(64), 0, st, p->d_scenes, table, map_view(p), df);
+    GL_HIP(hipGetLastError());
+  }
+  if (df.refine) {
+    hipLaunchKernelGGL(k_claim_order, order_grid, dim3(256), 0, st,
+                       reinterpret_cast<const unsigned char*>(df.refine),
+                       uint32_t(sizeof(RefineRecord)), uint32_t(offsetof(RefineRecord, bucket)),
+                       df.count + 2, df.refine_capacity, df.count + kRefineHist,
+                       df.count + kRefineFill, df.refine_order, lanes);
+    GL_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_refine_f64, grid, dim3(64), 0, st, p->d_scenes, table, map_view(p), df);
+    GL_HIP(hipGetLastError());
+  }
+  return GL_OK;
+}
+
 // The finisher over however many rays were deferred, then the refine pass
 // over however many were near-critical (both counts read on the device).
 // Timing events 2 and 3 bracket both.
@@ -885,18 +929,8 @@ gl_status launch_f64_passes(gl_trace_plan* p, const FrameScene* table, const Def
                             hipStream_t st) {
   if (!df.rec && !df.refine) return GL_OK;
   mark(p, 2, st);
-  const dim3 grid(std::max(1, p->cu_count * 4));
-  if (df.rec) {
-    hipLaunchKernelGGL(k_defer_order, dim3(std::max(1, p->cu_count)), dim3(256), 0, st, df,
-                       uint32_t(grid.x * 64u));
-    GL_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_finish_f64, grid, dim3(64), 0, st, p->d_scenes, table, map_view(p), df);
-    GL_HIP(hipGetLastError());
-  }
-  if (df.refine) {
-    hipLaunchKernelGGL(k_refine_f64, grid, dim3(64), 0, st, p->d_scenes, table, map_view(p), df);
-    GL_HIP(hipGetLastError());
-  }
+  gl_status s = launch_f64_kernels(p, table, df, st);
+  if (s) return s;
   mark(p, 3, st);
   return GL_OK;
 }
@@ -1348,7 +1382,7 @@ gl_status gl_geodesic_batch_trace(gl_trace_plan* p, uint32_t scene, const double
 // ---------------------------------------------------------------------------
 // Allocation-free geodesic batch. The frame record, the four workspace
 // counters and the deferral and refine records of up to kNoAllocMaxRays
-// requests live in one file-scope device array, `scratch` (about 6.3 MB,
+// requests live in one file-scope device array, `scratch` (about 6.8 MB,
 // held on each GPU the library runs on until the process ends). Its device
 // address is taken once per device with hipGetSymbolAddress; host code uses
 // only that address. Every call on a GPU shares the array: calls that
@@ -1361,7 +1395,10 @@ constexpr size_t kNoAllocCounters = align256(sizeof(FrameScene));
 constexpr size_t kNoAllocDefer = kNoAllocCounters + align256(kCounterWords * sizeof(unsigned int));
 constexpr size_t kNoAllocOrder = align256(kNoAllocDefer + kNoAllocMaxRays * sizeof(DeferRecord));
 constexpr size_t kNoAllocRefine = align256(kNoAllocOrder + kNoAllocMaxRays * sizeof(uint32_t));
-constexpr size_t kNoAllocBytes = align256(kNoAllocRefine + kNoAllocMaxRays * sizeof(RefineRecord));
+constexpr size_t kNoAllocRefineOrder =
+    align256(kNoAllocRefine + kNoAllocMaxRays * sizeof(RefineRecord));
+constexpr size_t kNoAllocBytes =
+    align256(kNoAllocRefineOrder + kNoAllocMaxRays * sizeof(uint32_t));
 static_assert(alignof(FrameScene) <= 256 && alignof(DeferRecord) <= 256 &&
                   alignof(RefineRecord) <= 256,
               "scratch regions are 256-byte aligned");
@@ -1426,6 +1463,7 @@ gl_status gl_geodesic_batch_trace_noalloc(gl_trace_plan* p, uint32_t scene, cons
   df.count = counters;
   df.capacity = f64 ? kNoAllocMaxRays : 0u;
   df.refine = refine ? reinterpret_cast<RefineRecord*>(b + kNoAllocRefine) : nullptr;
+  df.refine_order = refine ? reinterpret_cast<uint32_t*>(b + kNoAllocRefineOrder) : nullptr;
   df.refine_capacity = refine ? kNoAllocMaxRays : 0u;
   FrameScene f = make_frame(scene, 0, uint32_t(n), *out);
   f.states = states8;
@@ -1436,19 +1474,7 @@ gl_status gl_geodesic_batch_trace_noalloc(gl_trace_plan* p, uint32_t scene, cons
   hipLaunchKernelGGL(k_geodesic_batch, dim3(uint32_t((n + 63) / 64)), dim3(64), 0, st,
                      p->d_scenes, frame, states8, directions, uint32_t(n), map_view(p), df);
   GL_HIP(hipGetLastError());
-  const dim3 grid(std::max(1, p->cu_count * 4));
-  if (df.rec) {
-    hipLaunchKernelGGL(k_defer_order, dim3(std::max(1, p->cu_count)), dim3(256), 0, st, df,
-                       uint32_t(grid.x * 64u));
-    GL_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_finish_f64, grid, dim3(64), 0, st, p->d_scenes, frame, map_view(p), df);
-    GL_HIP(hipGetLastError());
-  }
-  if (df.refine) {
-    hipLaunchKernelGGL(k_refine_f64, grid, dim3(64), 0, st, p->d_scenes, frame, map_view(p), df);
-    GL_HIP(hipGetLastError());
-  }
-  return GL_OK;
+  return launch_f64_kernels(p, frame, df, st);
 }
 
 gl_status gl_trace_plan_queue_stats(gl_trace_plan* p, gl_queue_stats* stats, void* stream) {

@@ -62,6 +62,8 @@ enum {
  *              (turnings, crossings, winding), the travel time and the
  *              azimuth advance, which grow with the time spent near the
  *              critical orbit, are not settled; the class may still be
+ *              definite (the margin lies above near_critical_factor times
+ *              the budget)
  * A ray is GL_TERM_UNCERTAIN when its terminal events tie
  * (GL_FLAG_TIED_EVENTS), a radial turning lies inside the tie window of the
  * terminal (GL_FLAG_TURNING_ORDER), an invariant drift exceeds the scene
@@ -377,7 +379,7 @@ gl_status gl_geodesic_batch_trace_persistent(gl_trace_plan* plan, uint32_t scene
  * cannot allocate (code under HIP graph capture, tight serving loops,
  * harnesses that forbid hipMalloc). No counterpart in the reference. The
  * frame record, counters and f64 deferral/refine records live in a static
- * device array of about 6.3 MB per GPU (held until the process ends); the
+ * device array of about 6.8 MB per GPU (held until the process ends); the
  * caller passes the plan, inputs and outputs as for gl_geodesic_batch_trace,
  * and gets the same results. At most 32768 requests per call (else
  * GL_ERR_CAPACITY). Every call on one GPU shares that array: calls that
