@@ -120,34 +120,72 @@ __device__ __forceinline__ void image_state_f64(const SceneDev& s, uint32_t px, 
   y[7] = pu[3] + fl * g.lz;
 }
 
-// The f32 image ray: the rounding of the f64 state above. Out of line and
-// returned in registers (four dwords), so the ray state of the calling
-// kernel never has its address taken and stays in VGPRs.
-struct Momentum {
-  float px, py, pz, E;
-  float r_m;  // radius of the minimum of the ray's radial potential, 0 if untested
+// The start of the near-critical test (near_critical.h) from a ray's exact
+// f64 initial state y: the radius r_m of its radial potential's minimum
+// (0 when the scene does not test), the margin |mu| there, and the start
+// bits (kStartCritical, kStartBranch) when the f32 rounding of y alone
+// (f32_start_error; skipped for an exact f64 start) puts |mu| within the
+// near-critical or the branch factor. The trace carries these to its end,
+// where ray_verdict adds its own drift near r_m.
+struct StartTest {
+  float r_m;
+  uint32_t bits;  // kStartCritical | kStartBranch | the margin in kStartMuMask
 };
 
-// r_m of the exact ray from f64 state y for the near-critical test
-// (near_critical.h), 0 when the scene does not test.
-__device__ __forceinline__ float critical_radius(const SceneDev& s, const double* y) {
-  if (!(s.near_critical_factor > 0.0f) || !(s.r_horizon_d > 0.0)) return 0.0f;
-  const RadialMargin<double> m = state_margin(y, s.spin_d, s.charge2_d, double(s.chart), s.r_horizon_d);
-  return m.ok ? float(m.r_m) : 0.0f;
+__device__ __forceinline__ StartTest start_test(const SceneDev& s, const double* y, bool exact) {
+  StartTest t{0.0f, 0u};
+  if (!(s.near_critical_factor > 0.0f) || !(s.r_horizon_d > 0.0)) return t;
+  const MarginAndBound mb =
+      state_margin_and_bound(y, s.spin_d, s.charge2_d, double(s.chart), s.r_horizon_d);
+  const RadialMargin<double>& m = mb.m;
+  if (!m.ok) return t;
+  const double mu = fabs(m.mu);
+  t.r_m = float(m.r_m);
+  t.bits = __float_as_uint(float(mu)) & kStartMuMask;
+  // the start error can flag the ray only within the branch factor of its
+  // bound (start_error_bound_from >= f32_start_error); far from the critical
+  // curve it is not evaluated
+  if (!exact && mu <= double(s.branch_factor) * mb.bound) {
+    double e0 = f32_start_error(y, s.spin_d, s.charge2_d, double(s.chart));
+    if (!(e0 == e0)) e0 = 1.0;
+    if (mu <= double(s.near_critical_factor) * e0) t.bits |= kStartCritical;
+    if (mu <= double(s.branch_factor) * e0) t.bits |= kStartBranch;
+  }
+  return t;
 }
+
+// The f32 image ray: the rounding of the f64 state above, and its start
+// test. Out of line and returned in registers (six dwords), so the ray
+// state of the calling kernel never has its address taken and stays in
+// VGPRs.
+struct Momentum {
+  float px, py, pz, E;
+  StartTest start;
+};
 
 __device__ __noinline__ Momentum image_momentum(const SceneDev& s, uint32_t px, uint32_t py,
                                                 float ox, float oy) {
   double y[8];
   image_state_f64(s, px, py, double(ox), double(oy), y);
-  return Momentum{float(y[5]), float(y[6]), float(y[7]), float(-y[4]), critical_radius(s, y)};
+  return Momentum{float(y[5]), float(y[6]), float(y[7]), float(-y[4]), start_test(s, y, false)};
 }
 
-__device__ __noinline__ float batch_critical_radius(const SceneDev& s, const double* y8) {
+__device__ __noinline__ StartTest batch_start_test(const SceneDev& s, const double* y8) {
   double y[8];
 #pragma unroll
   for (int i = 0; i < 8; ++i) y[i] = y8[i];
-  return critical_radius(s, y);
+  return start_test(s, y, false);
+}
+
+template <class T>
+__device__ __forceinline__ void apply_start(RayT<T>& R, const StartTest& t) {
+  R.r_m = T(t.r_m);
+  R.armed |= t.bits;
+}
+
+// The margin |mu| of the start test, as the trace carried it.
+__device__ __forceinline__ float start_margin(uint32_t armed) {
+  return __uint_as_float(armed & kStartMuMask);
 }
 
 // The interval radial-capture certificate (certificate.h) of a backward ray
@@ -176,7 +214,7 @@ __device__ __forceinline__ void init_image_ray_at(const SceneDev& s, const Scene
   init_ray<float>(sc, R, s.obs_x, s.obs_y, s.obs_z, cx, cy, cz, m.px, m.py, m.pz, m.E, -1.0f,
                   s.polar_side);
   R.h = -s.h_init;
-  R.r_m = m.r_m;
+  apply_start(R, m.start);
   if ((sc.options & kOptCertificate) && R.term == kTermActive && R.flags == 0u &&
       certify_capture(s, s.obs_x, s.obs_y, s.obs_z, R.px, R.py, R.pz, R.E, R.r, R.rdot)) {
     R.term = kTermHorizon;
@@ -199,7 +237,7 @@ __device__ void init_state_ray(const SceneDev& s, const SceneT<float>& sc, const
   init_ray<float>(sc, R, x, yy, z, cx, cy, cz, float(y[5]), float(y[6]), float(y[7]),
                   float(-y[4]), dir, side);
   R.h = dir * s.h_init;
-  R.r_m = batch_critical_radius(s, y);
+  apply_start(R, batch_start_test(s, y));
   // backward requests are pixel rays in all but their start: the same
   // certificate applies (certificate.h: its own domain checks refuse what it
   // cannot prove, and the ray is traced in full)
@@ -409,33 +447,29 @@ struct Verdict {
   uint32_t extra;  // GL_STATUS_NEAR_CRITICAL, GL_STATUS_BRANCH_UNSETTLED or 0
 };
 
-// The near-critical test of a finished trace (near_critical.h): from the
-// exact f64 initial state, the radial-potential margin mu and its radius
-// r_m. A ray that came within 1.25 r_m (its smallest turning radius, or its
-// terminal radius, r_near) is measured against its error budget eps, the
-// largest drift it sampled at r >= r_m and, for an f32 trace, the rounding
-// of its initial constants: with |mu| <= near_critical_factor * eps the
-// capture/scatter decision is unsettled (GL_STATUS_NEAR_CRITICAL), with
-// |mu| <= branch_factor * eps the branch key and the time and azimuth spent
-// near r_m are (GL_STATUS_BRANCH_UNSETTLED). Out of line and with scalar
-// arguments, once per ray.
-__device__ __noinline__ Verdict ray_verdict(const SceneDev& s, const double* state8, uint32_t px,
-                                            uint32_t py, float r_near, float drift,
-                                            bool exact_start) {
-  double y[8];
-  source_state(s, state8, px, py, y);
+// The near-critical test of a finished trace (near_critical.h), from the
+// start test its trace carried (r_m, |mu| and the start bits, from the exact
+// f64 initial state): a ray that came within 1.25 r_m (its smallest turning
+// radius, or its terminal radius, r_near) is measured against its error
+// budget, the largest drift it sampled at r in [r_m, 2 r_m] and, for an f32
+// trace, the rounding of its initial constants (the start bits): with
+// |mu| <= near_critical_factor * budget the capture/scatter decision is
+// unsettled (GL_STATUS_NEAR_CRITICAL), with |mu| <= branch_factor * budget
+// the branch key and the time and azimuth spent near r_m are
+// (GL_STATUS_BRANCH_UNSETTLED). phi0 is the azimuth of the initial position.
+__device__ __noinline__ Verdict ray_verdict(const SceneDev& s, const double* state8,
+                                            float r_near, float drift, float r_m, float mu,
+                                            uint32_t start_bits) {
   Verdict v;
-  v.phi0 = ::atan2(y[2], y[1]);
+  v.phi0 = state8 ? ::atan2(state8[2], state8[1]) : s.phi_observer_d;
   v.extra = 0u;
-  if (!(s.near_critical_factor > 0.0f) || !(s.r_horizon_d > 0.0)) return v;
-  const RadialMargin<double> m = state_margin(y, s.spin_d, s.charge2_d, double(s.chart), s.r_horizon_d);
-  if (!m.ok || !(double(r_near) <= 1.25 * m.r_m)) return v;
-  double eps = double(drift);
-  if (!exact_start) eps = fmax(eps, f32_start_error(y, s.spin_d, s.charge2_d, double(s.chart)));
-  if (!(eps == eps)) eps = 1.0;
-  const double mu = fabs(m.mu);
-  if (mu <= double(s.near_critical_factor) * eps) v.extra |= GL_STATUS_NEAR_CRITICAL;
-  if (mu <= double(s.branch_factor) * eps) v.extra |= GL_STATUS_BRANCH_UNSETTLED;
+  if (!(s.near_critical_factor > 0.0f) || !(r_m > 0.0f) || !(r_near <= 1.25f * r_m)) return v;
+  float eps = drift;
+  if (!(eps == eps)) eps = 1.0f;
+  if (mu <= s.near_critical_factor * eps || (start_bits & kStartCritical))
+    v.extra |= GL_STATUS_NEAR_CRITICAL;
+  if (mu <= s.branch_factor * eps || (start_bits & kStartBranch))
+    v.extra |= GL_STATUS_BRANCH_UNSETTLED;
   return v;
 }
 
@@ -557,7 +591,8 @@ __device__ __forceinline__ bool write_ray(const RayT<float>& R, const OutPlanes&
   const unsigned long long n = o.count;
   if (idx >= n) return true;
   const float r_near = R.r_turn >= 0.0f ? fminf(R.r_turn, R.r) : R.r;
-  const Verdict v = ray_verdict(s, src.state8, src.px, src.py, r_near, R.d_crit, refined);
+  const Verdict v = ray_verdict(s, src.state8, r_near, R.d_crit, R.r_m, start_margin(R.armed),
+                                R.armed);
   const bool definite = event_class(R.term);
   const bool critical = definite && (v.extra & GL_STATUS_NEAR_CRITICAL);
   // the refine pass settles the branch as well as the class

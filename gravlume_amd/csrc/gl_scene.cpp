@@ -453,6 +453,7 @@ int prepare_scene(const gl_scene_desc& d, SceneDev* out, SceneHost* host, std::s
   s.sub_x_d = d.subpixel[0];
   s.sub_y_d = d.subpixel[1];
   s.r_observer_d = r_obs;
+  s.phi_observer_d = std::atan2(ox[1], ox[0]);
   s.near_critical_factor = d.near_critical_factor == 0.0 ? 16.0f
                            : (d.near_critical_factor < 0.0 ? -1.0f : (float)d.near_critical_factor);
   s.branch_factor = s.near_critical_factor < 0.0f ? -1.0f

@@ -384,6 +384,7 @@ __device__ __forceinline__ RayT<float> narrow_ray(const RayT<double>& R) {
   W.pz = float(R.pz);
   W.E = float(R.E);
   W.term = R.term;
+  W.armed = R.armed;
   W.candidates = R.candidates;
   W.flags = R.flags;
   W.accepted = R.accepted;
@@ -590,7 +591,7 @@ __device__ __forceinline__ void start_refined(const SceneDev* __restrict__ scene
   const uint32_t side = f.states ? (y[3] > 0.0 ? 2u : (y[3] < 0.0 ? 0u : 1u)) : s.polar_side;
   init_ray<double>(sd, R, y[1], y[2], y[3], 0.0, 0.0, 0.0, y[5], y[6], y[7], -y[4], dir, side);
   R.h = double(dir) * double(s.h_init);
-  R.r_m = critical_radius(s, y);
+  apply_start(R, start_test(s, y, true));  // an exact start: no rounding bits
 }
 
 // The f64 refine pass (GL_OPT_NEAR_CRITICAL_REFINE): every near-critical

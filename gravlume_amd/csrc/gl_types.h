@@ -63,6 +63,7 @@ struct alignas(16) SceneDev {
   double right_d[4], up_d[4], arrival_d[4];
   double tan_half_fov_d, aspect_d, sub_x_d, sub_y_d;
   double r_observer_d;  // oblate radius of the observer
+  double phi_observer_d;  // azimuth atan2(y, x) of the observer
   // near-critical test and its f64 refine (gl_scene_desc)
   float near_critical_factor;  // <= 0: no test
   float branch_factor;

@@ -51,19 +51,20 @@ GL_HD RadialMargin<T> radial_margin(T E, T L, T Q, T spin, T charge2, T r_plus, 
   const T lm = L - a * E;
   const T B = lm * lm + Q;
   const T E2 = E * E;
-  const T p = (T(2) * E * c - B) / (T(2) * E2);
-  const T s = B / (T(2) * E2);
+  const T inv2e2 = T(1) / (T(2) * E2);
+  const T p = (T(2) * E * c - B) * inv2e2;
+  const T s = B * inv2e2;
   T r = r_plus;
   if (p < T(0) && T(4) * p * p * p + T(27) * s * s < T(0)) {
     // largest root of r^3 + p r + s: Newton from 2 sqrt(-p/3), above it, where
     // the cubic is positive, increasing and convex, so the iterates decrease
     // monotonically onto the root
-    T x = T(2) * gl_sqrt<T>(-p / T(3));
+    T x = T(2) * gl_sqrt<T>(p * T(-1.0 / 3.0));
     for (int it = 0; it < 60; ++it) {
       const T f = (x * x + p) * x + s;
       const T df = T(3) * x * x + p;
       if (!(df > T(0))) break;
-      const T nx = x - f / df;
+      const T nx = x - f * gl_rcp<T>(df);
       if (!(nx < x)) break;
       x = nx;
     }
@@ -106,6 +107,78 @@ GL_HD double f32_start_error(const double* y, double spin, double charge2, doubl
 
 // The margin of the exact ray from f64 initial state y, with the scene's
 // spacetime (no horizon: not near-critical).
+// An upper bound of f32_start_error(y) from first-order error propagation:
+// rounding E and p to binary32 moves each by at most u = 2^-24 of itself, so
+//   |dL| <= u (|x p_y| + |y p_x|),
+//   |dQ| <= 2 u (cz^2 D (p_x^2 + p_y^2) + cz^2 a^2 E^2 + 2 |z| (|x p_x| + |y p_y|) |p_z|
+//               + r^2 (x^2 + y^2) p_z^2 / D)   (Carter's terms, quadratic in p),
+// and the null contraction -E t' + p . x' of the rounded momentum is 2 H
+// there (Euler: p . dH/dp = 2 H for the quadratic H), which exceeds the
+// state's own 2 H by at most 2 u times the contraction's term norm, so the
+// null residual is at most |2 H| + 2 u, with 2 H = -E^2 + |p|^2 - f q^2 and
+// q = E + l . p (zero for a null state). A 1% margin covers the
+// second-order terms and the f64 evaluation; the absolute 1e-30 covers
+// binary32 subnormals. When |mu| exceeds the branch
+// factor times this bound, the start error cannot flag the ray and need not
+// be evaluated.
+GL_HD double start_error_bound_from(const Geom<double>& g, double x, double yy, double z,
+                                    double E, double px, double py, double pz, double L,
+                                    double Q, double a2) {
+  const double u = 5.9604644775390625e-08;  // 2^-24
+  const double D = g.r * g.r + a2;
+  const double cz = z * g.rinv;
+  const double axpx = gl_abs<double>(x * px), aypy = gl_abs<double>(yy * py);
+  const double tq = cz * cz * D * (px * px + py * py) + cz * cz * a2 * E * E +
+                    2.0 * gl_abs<double>(z) * (axpx + aypy) * gl_abs<double>(pz) +
+                    g.r * g.r * (x * x + yy * yy) * g.kinv * pz * pz;
+  const double el = u * (gl_abs<double>(x * py) + gl_abs<double>(yy * px)) /
+                    (gl_abs<double>(L) > 1.0 ? gl_abs<double>(L) : 1.0);
+  const double eq = 2.0 * u * tq / (gl_abs<double>(Q) > 1.0 ? gl_abs<double>(Q) : 1.0);
+  const double q = E + g.lx * px + g.ly * py + g.lz * pz;
+  const double h2 = -E * E + px * px + py * py + pz * pz - g.f * q * q;
+  const double scale2 = E * E + px * px + py * py + pz * pz + gl_abs<double>(g.f) * q * q;
+  const double en = gl_abs<double>(h2) + 2.0 * u + 1.0e-14 * scale2;
+  double e = el > eq ? el : eq;
+  if (e < en) e = en;
+  return 1.01 * e + 1.0e-30;
+}
+
+GL_HD double f32_start_error_bound(const double* y, double spin, double charge2, double chart) {
+  const double x = y[1], yy = y[2], z = y[3];
+  const double E = -y[4], px = y[5], py = y[6], pz = y[7];
+  const double a2 = spin * spin;
+  const Geom<double> g = geometry<double>(x, yy, z, 1.0, spin, a2, charge2, chart);
+  if (!g.ok) return 1.0;
+  const double L = x * py - yy * px;
+  const double Q = carter<double>(g, x, yy, z, px, py, pz, E, a2);
+  return start_error_bound_from(g, x, yy, z, E, px, py, pz, L, Q, a2);
+}
+
+// The margin of state y with the bound above, from one geometry evaluation.
+struct MarginAndBound {
+  RadialMargin<double> m;
+  double bound;
+};
+
+GL_HD MarginAndBound state_margin_and_bound(const double* y, double spin, double charge2,
+                                            double chart, double r_plus) {
+  const double x = y[1], yy = y[2], z = y[3];
+  const double E = -y[4], px = y[5], py = y[6], pz = y[7];
+  const double a2 = spin * spin;
+  const Geom<double> g = geometry<double>(x, yy, z, 1.0, spin, a2, charge2, chart);
+  MarginAndBound r;
+  r.m.mu = 0.0;
+  r.m.r_m = r_plus;
+  r.m.ok = false;
+  r.bound = 1.0;
+  if (!g.ok) return r;
+  const double L = x * py - yy * px;
+  const double Q = carter<double>(g, x, yy, z, px, py, pz, E, a2);
+  r.m = radial_margin<double>(E, L, Q, spin, charge2, r_plus, g.r);
+  r.bound = start_error_bound_from(g, x, yy, z, E, px, py, pz, L, Q, a2);
+  return r;
+}
+
 GL_HD RadialMargin<double> state_margin(const double* y, double spin, double charge2, double chart,
                                         double r_plus) {
   const double x = y[1], yy = y[2], z = y[3];

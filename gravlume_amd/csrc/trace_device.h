@@ -207,6 +207,14 @@ enum : uint32_t {
   kTermCertified = 9,
 };
 enum : uint32_t { kEvSing = 1u, kEvHor = 2u, kEvSurf = 4u, kEvEsc = 8u };
+// Bits of RayT::armed above the event bits, set once at the ray's start by
+// the near-critical test: kStartCritical / kStartBranch when the f32
+// rounding of the initial state alone puts the margin within the
+// near-critical / the branch factor of the error budget, and in bits 16-31
+// the margin |mu| itself as the upper half of its binary32 pattern (7
+// mantissa bits, rounded toward zero: the stored margin errs on the flagging
+// side). Kept there, the margin costs the trace loop no register.
+enum : uint32_t { kStartCritical = 1u << 8, kStartBranch = 1u << 9, kStartMuMask = 0xffff0000u };
 enum : uint32_t {
   kFlagNonFinite = 1u,
   kFlagRadicand = 2u,
@@ -246,9 +254,10 @@ struct RayT {
   // the turning step (enough for the near-critical test's proximity check).
   T r_turn;
   // The near-critical test (near_critical.h): r_m, the radius of the minimum
-  // of the ray's radial potential (0: no test), and d_crit, the largest
-  // relative drift sampled at r in [r_m, 2 r_m], where the capture/scatter
-  // decision is taken.
+  // of the ray's radial potential (0: no test), from the exact initial state
+  // (its margin there rides in the high bits of `armed`), and d_crit, the
+  // largest relative drift sampled at r in [r_m, 2 r_m], where the
+  // capture/scatter decision is taken.
   T r_m, d_crit;
   uint32_t accepted, rejected, consec;
   uint32_t turnings, crossings;

@@ -30,3 +30,8 @@ extern "C" double nc_f32_start_error(const double* state8, double spin, double c
                                      double chart) {
   return gl::f32_start_error(state8, spin, charge2, chart);
 }
+
+extern "C" double nc_f32_start_error_bound(const double* state8, double spin, double charge2,
+                                           double chart) {
+  return gl::f32_start_error_bound(state8, spin, charge2, chart);
+}
