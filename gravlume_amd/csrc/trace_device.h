@@ -441,7 +441,10 @@ GL_HD uint32_t arm_events(const SceneT<T>& sc, T r, T z, uint32_t armed) {
   if (sc.r_capture > T(0) && r - sc.r_capture > sc.band) armed |= kEvHor;
   if (r - sc.r_escape < -sc.band) armed |= kEvEsc;
   if (sc.disk_in > T(0) && gl_abs<T>(z) > sc.band) armed |= kEvSurf;
-  if (singularity_measure<T>(r, sc.spin2, z) - sc.guard > sc.band) armed |= kEvSing;
+  // once armed, an event stays armed: the singularity measure is formed only
+  // until then
+  if (!(armed & kEvSing) && singularity_measure<T>(r, sc.spin2, z) - sc.guard > sc.band)
+    armed |= kEvSing;
   return armed;
 }
 
@@ -905,7 +908,8 @@ GL_HD void step(const SceneT<T>& sc, RayT<T>& R, T omega, bool reproject = false
 
   // Terminal event candidates (fraction > 1 means none).
   T fr[4] = {T(2), T(2), T(2), T(2)};
-  if (R.armed & kEvSing) {
+  // the measure r^4 + a^2 z^2 falls to the guard only where r^4 <= guard
+  if ((R.armed & kEvSing) && (r1 * r1) * (r1 * r1) <= sc.guard) {
     T m0 = singularity_measure<T>(r0, sc.spin2, z0) - sc.guard;
     T m1 = singularity_measure<T>(r1, sc.spin2, z1) - sc.guard;
     if (m0 > T(0) && m1 <= T(0)) fr[0] = gl_clamp<T>(m0 * gl_rcp<T>(m0 - m1), T(0), T(1));
