@@ -128,20 +128,19 @@ __device__ __forceinline__ void image_state_f64(const SceneDev& s, uint32_t px, 
 // near-critical or the branch factor. The trace carries these to its end,
 // where ray_verdict adds its own drift near r_m.
 struct StartTest {
-  float r_m;
-  uint32_t bits;  // kStartCritical | kStartBranch | the margin in kStartMuMask
+  uint32_t bits;  // kStartCritical | kStartBranch | r_m and |mu| (trace_device.h)
 };
 
 __device__ __forceinline__ StartTest start_test(const SceneDev& s, const double* y, bool exact) {
-  StartTest t{0.0f, 0u};
+  StartTest t{0u};
   if (!(s.near_critical_factor > 0.0f) || !(s.r_horizon_d > 0.0)) return t;
   const MarginAndBound mb =
       state_margin_and_bound(y, s.spin_d, s.charge2_d, double(s.chart), s.r_horizon_d);
   const RadialMargin<double>& m = mb.m;
   if (!m.ok) return t;
   const double mu = fabs(m.mu);
-  t.r_m = float(m.r_m);
-  t.bits = __float_as_uint(float(mu)) & kStartMuMask;
+  t.bits = (encode_start_radius(float(m.r_m)) << kStartRmShift) |
+           (encode_start_margin(float(mu)) << kStartMuShift);
   // the start error can flag the ray only within the branch factor of its
   // bound (start_error_bound_from >= f32_start_error); far from the critical
   // curve it is not evaluated
@@ -155,13 +154,14 @@ __device__ __forceinline__ StartTest start_test(const SceneDev& s, const double*
 }
 
 // The f32 image ray: the rounding of the f64 state above, and its start
-// test. Out of line and returned in registers (six dwords), so the ray
+// test. Out of line and returned in registers (five dwords), so the ray
 // state of the calling kernel never has its address taken and stays in
 // VGPRs.
 struct Momentum {
   float px, py, pz, E;
   StartTest start;
 };
+static_assert(sizeof(Momentum) == 20, "five dwords in registers");
 
 __device__ __noinline__ Momentum image_momentum(const SceneDev& s, uint32_t px, uint32_t py,
                                                 float ox, float oy) {
@@ -179,13 +179,7 @@ __device__ __noinline__ StartTest batch_start_test(const SceneDev& s, const doub
 
 template <class T>
 __device__ __forceinline__ void apply_start(RayT<T>& R, const StartTest& t) {
-  R.r_m = T(t.r_m);
   R.armed |= t.bits;
-}
-
-// The margin |mu| of the start test, as the trace carried it.
-__device__ __forceinline__ float start_margin(uint32_t armed) {
-  return __uint_as_float(armed & kStartMuMask);
 }
 
 // The interval radial-capture certificate (certificate.h) of a backward ray
@@ -458,17 +452,18 @@ struct Verdict {
 // the branch key and the time and azimuth spent near r_m are
 // (GL_STATUS_BRANCH_UNSETTLED). phi0 is the azimuth of the initial position.
 __device__ __noinline__ Verdict ray_verdict(const SceneDev& s, const double* state8,
-                                            float r_near, float drift, float r_m, float mu,
-                                            uint32_t start_bits) {
+                                            float r_near, float drift, uint32_t armed) {
   Verdict v;
   v.phi0 = state8 ? ::atan2(state8[2], state8[1]) : s.phi_observer_d;
   v.extra = 0u;
+  const float r_m = decode_start_radius(armed);
   if (!(s.near_critical_factor > 0.0f) || !(r_m > 0.0f) || !(r_near <= 1.25f * r_m)) return v;
+  const float mu = decode_start_margin(armed);
   float eps = drift;
   if (!(eps == eps)) eps = 1.0f;
-  if (mu <= s.near_critical_factor * eps || (start_bits & kStartCritical))
+  if (mu <= s.near_critical_factor * eps || (armed & kStartCritical))
     v.extra |= GL_STATUS_NEAR_CRITICAL;
-  if (mu <= s.branch_factor * eps || (start_bits & kStartBranch))
+  if (mu <= s.branch_factor * eps || (armed & kStartBranch))
     v.extra |= GL_STATUS_BRANCH_UNSETTLED;
   return v;
 }
@@ -591,8 +586,7 @@ __device__ __forceinline__ bool write_ray(const RayT<float>& R, const OutPlanes&
   const unsigned long long n = o.count;
   if (idx >= n) return true;
   const float r_near = R.r_turn >= 0.0f ? fminf(R.r_turn, R.r) : R.r;
-  const Verdict v = ray_verdict(s, src.state8, r_near, R.d_crit, R.r_m, start_margin(R.armed),
-                                R.armed);
+  const Verdict v = ray_verdict(s, src.state8, r_near, R.d_crit, R.armed);
   const bool definite = event_class(R.term);
   const bool critical = definite && (v.extra & GL_STATUS_NEAR_CRITICAL);
   // the refine pass settles the branch as well as the class
@@ -664,7 +658,7 @@ __device__ __forceinline__ bool try_defer(const DeferBuf& df, const RayT<float>&
   df.rec[slot].ray = R;
   df.rec[slot].frame_scene = fs;
   df.rec[slot].index = idx;
-  const uint32_t b = defer_bucket(R.r_m);
+  const uint32_t b = defer_bucket(decode_start_radius(R.armed));
   df.rec[slot].bucket = b;
   atomicAdd(df.count + kDeferHist + b, 1u);
   return true;

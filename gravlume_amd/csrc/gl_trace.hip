@@ -218,7 +218,10 @@ __global__ void __launch_bounds__(256, kWaves) k_persistent_tiles(
     if (base >= total) break;
     ++tiles;
     const unsigned long long item = base + lane;
-    const uint32_t fs = kSingle ? 0u : find_frame_scene(frame, n_frame, item);
+    // the single-scene variant looks its frame slice up per tile as well:
+    // with a constant slice the compiler keeps the scene's packed constants
+    // live across the tile loop, registers the trace loop needs (it spilled)
+    const uint32_t fs = find_frame_scene(frame, n_frame, item);
     const FrameScene& f = frame[fs];
     const SceneDev& s = scenes[f.scene];
     SceneT<float> sc = scene_f32(s);
@@ -402,7 +405,6 @@ __device__ __forceinline__ RayT<float> narrow_ray(const RayT<double>& R) {
   W.winding = R.winding;
   W.polar_side = R.polar_side;
   W.r_turn = float(R.r_turn);
-  W.r_m = float(R.r_m);
   W.d_crit = float(R.d_crit);
   return W;
 }
@@ -441,7 +443,6 @@ __device__ __forceinline__ void widen_deferred(const RayT<float>& Rf, RayT<doubl
   R.polar_side = Rf.polar_side;
   R.dir = Rf.dir;
   R.r_turn = Rf.r_turn;
-  R.r_m = Rf.r_m;
   R.d_crit = Rf.d_crit;
 }
 

@@ -208,13 +208,62 @@ enum : uint32_t {
 };
 enum : uint32_t { kEvSing = 1u, kEvHor = 2u, kEvSurf = 4u, kEvEsc = 8u };
 // Bits of RayT::armed above the event bits, set once at the ray's start by
-// the near-critical test: kStartCritical / kStartBranch when the f32
-// rounding of the initial state alone puts the margin within the
-// near-critical / the branch factor of the error budget, and in bits 16-31
-// the margin |mu| itself as the upper half of its binary32 pattern (7
-// mantissa bits, rounded toward zero: the stored margin errs on the flagging
-// side). Kept there, the margin costs the trace loop no register.
-enum : uint32_t { kStartCritical = 1u << 8, kStartBranch = 1u << 9, kStartMuMask = 0xffff0000u };
+// the near-critical test (near_critical.h) from its exact initial state:
+// kStartCritical / kStartBranch when the f32 rounding of the initial state
+// alone puts the margin within the near-critical / the branch factor of the
+// error budget; bits 6-18 the radius r_m of the radial potential's minimum
+// and bits 19-31 the margin |mu| there, each as a 13-bit float (r_m: 5
+// exponent bits for [2^-2, 2^29), 8 mantissa bits, rounded up, 0 = no test;
+// |mu|: 6 exponent bits for [2^-60, 4), 7 mantissa bits, rounded toward
+// zero, 0 below 2^-60). The roundings err on the flagging side, and carried
+// this way the test costs the trace loop no register.
+enum : uint32_t { kStartCritical = 1u << 4, kStartBranch = 1u << 5 };
+constexpr uint32_t kStartRmShift = 6u, kStartMuShift = 19u, kStartFieldMask = 0x1fffu;
+
+GL_HD uint32_t gl_f2u(float v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __float_as_uint(v);
+#else
+  uint32_t u;
+  __builtin_memcpy(&u, &v, 4);
+  return u;
+#endif
+}
+GL_HD float gl_u2f(uint32_t u) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __uint_as_float(u);
+#else
+  float v;
+  __builtin_memcpy(&v, &u, 4);
+  return v;
+#endif
+}
+
+// r_m as a 13-bit float, rounded up (0 for no test or a non-positive r_m).
+GL_HD uint32_t encode_start_radius(float r_m) {
+  if (!(r_m > 0.0f)) return 0u;
+  const float v = r_m < 0.25f ? 0.25f : (r_m > 5.0e8f ? 5.0e8f : r_m);
+  const uint32_t b = gl_f2u(v) + ((1u << 15) - 1u);  // round the mantissa up
+  return ((((b >> 23) & 255u) - 124u) << 8) | ((b >> 15) & 255u);
+}
+GL_HD float decode_start_radius(uint32_t armed) {
+  const uint32_t c = (armed >> kStartRmShift) & kStartFieldMask;
+  if (c == 0u) return 0.0f;
+  return gl_u2f((((c >> 8) + 124u) << 23) | ((c & 255u) << 15));
+}
+
+// |mu| as a 13-bit float, rounded toward zero (0 below 2^-60).
+GL_HD uint32_t encode_start_margin(float mu) {
+  if (!(mu >= 8.673617379884035e-19f)) return 0u;  // 2^-60, and NaN
+  const float v = mu > 3.99f ? 3.99f : mu;
+  const uint32_t b = gl_f2u(v);
+  return ((((b >> 23) & 255u) - 66u) << 7) | ((b >> 16) & 127u);
+}
+GL_HD float decode_start_margin(uint32_t armed) {
+  const uint32_t c = (armed >> kStartMuShift) & kStartFieldMask;
+  if (c == 0u) return 0.0f;
+  return gl_u2f((((c >> 7) + 66u) << 23) | ((c & 127u) << 16));
+}
 enum : uint32_t {
   kFlagNonFinite = 1u,
   kFlagRadicand = 2u,
@@ -253,12 +302,10 @@ struct RayT {
   // dense output with kOptTurningRadius, else the smaller endpoint radius of
   // the turning step (enough for the near-critical test's proximity check).
   T r_turn;
-  // The near-critical test (near_critical.h): r_m, the radius of the minimum
-  // of the ray's radial potential (0: no test), from the exact initial state
-  // (its margin there rides in the high bits of `armed`), and d_crit, the
-  // largest relative drift sampled at r in [r_m, 2 r_m], where the
-  // capture/scatter decision is taken.
-  T r_m, d_crit;
+  // The near-critical test (near_critical.h): d_crit, the largest relative
+  // drift sampled at r in [r_m, 2 r_m], where the capture/scatter decision
+  // is taken; r_m and the margin ride in the high bits of `armed`.
+  T d_crit;
   uint32_t accepted, rejected, consec;
   uint32_t turnings, crossings;
   int32_t winding;
@@ -474,7 +521,6 @@ GL_HD void init_ray(const SceneT<T>& sc, RayT<T>& R, T x, T y, T z, T cx, T cy, 
   R.flags = 0;
   R.polar_side = polar_side;
   R.r_turn = T(-1);
-  R.r_m = T(0);
   R.d_crit = T(0);
   if (!refresh_derivative<T>(sc, R)) {
     R.term = kTermFailure;
@@ -712,7 +758,8 @@ GL_HD void update_drift(const SceneT<T>& sc, RayT<T>& R, const Geom<T>& g, const
   // decision is taken, r in [r_m, 2 r_m]: beyond, the invariants' own
   // evaluation (terms of size r^2 p^2 cancelling to Carter's constant) is
   // the larger error, and below r_m a captured ray is already decided
-  if (g.r >= R.r_m && g.r <= T(2) * R.r_m)
+  const T r_m = T(decode_start_radius(R.armed));
+  if (g.r >= r_m && g.r <= T(2) * r_m)
     R.d_crit = gl_max<T>(R.d_crit, gl_max<T>(nr, gl_max<T>(dl, dq)));
   if (!(R.d_null <= sc.drift_limit && R.d_lz <= sc.drift_limit && R.d_q <= sc.drift_limit))
     R.flags |= kFlagDrift;
