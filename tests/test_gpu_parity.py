@@ -355,7 +355,12 @@ def test_geodesic_batch_reproduces_the_b6_fixture(gl):
 
 
 def test_frame_graph_replays_match_eager(gl):
-    from gravlume_amd import scene as S
+    """Graph replays of a persistent frame equal the eager frame, with eager
+    work on the same plan between them: a frame, and a geodesic batch four
+    times the frame's size whose f64 deferrals make the plan reallocate its
+    own deferral workspace (the graph replays into its own)."""
+    from gravlume_amd import critical as Cr, scene as S
+    import math
     import torch
     sc = S.single_frame_1080p(width=256, height=144)
     pipe = gl.TracePipeline([sc])
@@ -365,16 +370,26 @@ def test_frame_graph_replays_match_eager(gl):
     he = eager.to_host()
     tgt = pipe.create_target(0)
     graph = pipe.frame_graph([tgt])
-    for _ in range(3):
+    n = 4 * sc.rays
+    states, *_ = Cr.near_critical_states(sc.spin, sc.charge, sc.observer_radius,
+                                         math.radians(sc.inclination_deg), n, seed=12,
+                                         chart=sc.chart)
+    d_states = torch.from_numpy(states).cuda()
+    d_dirs = -torch.ones(n, dtype=torch.float32, device="cuda")
+    batch_out = gl.TraceTarget(n)
+    for rep in range(3):
         tgt.status.zero_()
         torch.cuda.synchronize()
         graph.launch()
         graph.stream.synchronize()
         h = tgt.to_host()
         for k in he:
-            assert np.array_equal(np.asarray(h[k]), np.asarray(he[k])), k
-        pipe.encode_persistent([eager])  # eager work between replays
+            assert np.array_equal(np.asarray(h[k]), np.asarray(he[k])), (rep, k)
+        # eager work between replays: a frame, then the larger batch
+        pipe.encode_persistent([eager])
+        gl.GeodesicBatch(pipe, 0).encode(d_states, d_dirs, batch_out)
         torch.cuda.synchronize()
+    assert np.all(batch_out.to_host()["termination"] > 0)
 
 
 def test_capture_certificate_on_the_default_kerr_view(gl, oracle):
