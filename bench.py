@@ -239,7 +239,9 @@ def near_critical_leg(g, S, torch, n=1_000_000, sample=1024, seed=20261208):
     1.05 r_+, the interval capture certificate enabled. Three runs of the
     same batch: the plain batch kernel and the persistent lane-refill queue
     (rays the f32 trace cannot settle leave Uncertain), and the persistent
-    queue with the f64 refine pass at rtol 1e-12 (the oracle's tolerance).
+    queue with the f64 refine pass at rtol 1e-12 (the oracle's tolerance);
+    then the whole batch in f64 at rtol 1e-12 (gl_geodesic_batch_trace_f64),
+    beside the oracle's own rate at that tolerance on the sample.
     Against the f64 oracle at rtol 1e-12 on a seeded sample, by |delta|
     band: the Uncertain share, the class agreement of definite rays, and the
     winding and travel time of definite rays whose branch is settled."""
@@ -282,6 +284,26 @@ def near_critical_leg(g, S, torch, n=1_000_000, sample=1024, seed=20261208):
                                             "a = 0.998 every ray falls back to the full trace"}
         hosts[name] = h
         del pipe, t
+    # the whole batch in f64 at the oracle's tolerance (gl_geodesic_batch_trace_f64)
+    sc64 = S.near_critical_scene(capture_certificate=True)
+    pipe = g.TracePipeline([sc64])
+    pipe.enable_timing(True)
+    batch = g.GeodesicBatch(pipe, 0)
+    t = g.TraceTarget(n)
+    batch.encode(d_states, d_dirs, t, f64_tolerance=(1e-12, 1e-14))  # warm-up
+    torch.cuda.synchronize()
+    pipe.kernel_ms()
+    t0 = time.perf_counter()
+    batch.encode(d_states, d_dirs, t, f64_tolerance=(1e-12, 1e-14))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    main_ms, _, _ = pipe.kernel_ms()
+    hosts["f64_batch"] = t.to_host()
+    res["f64_batch"] = {"rays_per_s": n / wall, "ms": 1e3 * wall, "kernel_ms": main_ms,
+                        "uncertain": int(np.sum(hosts["f64_batch"]["termination"] == 6)),
+                        "note": "every request in f64 DP5 at rtol 1e-12 atol 1e-14 from its exact "
+                                "initial state"}
+    del pipe, t
     res["persistent_over_plain"] = res["persistent"]["rays_per_s"] / res["plain"]["rays_per_s"]
     h = hosts["persistent"]
     res["steps"] = {"accepted_mean": float(h["steps"].mean()), "accepted_max": int(h["steps"].max()),
@@ -296,8 +318,10 @@ def near_critical_leg(g, S, torch, n=1_000_000, sample=1024, seed=20261208):
     st = O.spacetime(1.0, sc.spin, sc.charge, float(sc.chart))
     ev = O.events(sc.escape_radius, None, horizon_offset=sc.horizon_offset)
     dirs = -np.ones(sample)
+    t0 = time.perf_counter()
     ref = O.outcomes_to_arrays(O.trace_ordered(st, O.config_policy(1e-12, 1e-14), ev, states[idx],
                                                dirs, threads=cores))
+    t_ref = time.perf_counter() - t0
     fine = O.outcomes_to_arrays(O.trace_ordered(st, O.config_policy(1e-12 / 16, 1e-14 / 16), ev,
                                                 states[idx], dirs, threads=cores))
     t0 = time.perf_counter()
@@ -305,11 +329,12 @@ def near_critical_leg(g, S, torch, n=1_000_000, sample=1024, seed=20261208):
     t_work = time.perf_counter() - t0
     res["cpu_baseline_sample"] = {"rays_per_s": sample / t_work, "cores": cores, "kind": "port",
                                   "sample": f"{sample} seeded requests of the batch, f64 DP5 rtol "
-                                            f"{sc.rtol:g} atol {sc.atol:g}, 0.5 M step cap"}
+                                            f"{sc.rtol:g} atol {sc.atol:g}, 0.5 M step cap",
+                                  "rays_per_s_at_rtol_1e-12": sample / t_ref}
     ok = ref["termination"] == fine["termination"]
     ad = np.abs(delta[idx])
     cmp = {}
-    for name in ("persistent", "persistent_refined"):
+    for name in ("persistent", "persistent_refined", "f64_batch"):
         hh = hosts[name]
         t = hh["termination"][idx]
         bands = {}

@@ -143,6 +143,8 @@ SIGNATURES = {
                                           C.POINTER(RayOutputsC), _P]),
     "gl_geodesic_batch_trace_noalloc": (C.c_int, [_P, C.c_uint32, _P, _P, C.c_size_t,
                                                   C.POINTER(RayOutputsC), _P]),
+    "gl_geodesic_batch_trace_f64": (C.c_int, [_P, C.c_uint32, _P, _P, C.c_size_t, C.c_double,
+                                              C.c_double, C.POINTER(RayOutputsC), _P]),
     "gl_scene_validate": (C.c_int, [C.POINTER(SceneDesc)]),
     "gl_blackbody_log2_fractions": (C.c_int, [C.c_double, C.POINTER(C.c_double)]),
     "gl_blackbody_lut": (C.c_int, [C.POINTER(C.c_float), C.c_size_t]),

@@ -30,8 +30,10 @@ struct DeferRecord {
 // near-critical rays for the f64 refine pass, their claim orders and their
 // counters (kCounterWords words): count[0] deferred records, [1] the
 // finisher's claim cursor, [2] refine records, [3] the refine pass's claim
-// cursor, [kDeferHist + b] / [kRefineHist + b] records in claim bucket b,
+// cursor, [kBatchCursor] the f64 batch's claim cursor,
+// [kDeferHist + b] / [kRefineHist + b] records in claim bucket b,
 // [kDeferFill + b] / [kRefineFill + b] the ordering passes' cursors.
+constexpr uint32_t kBatchCursor = 4u;  // the f64 geodesic batch's claim cursor
 constexpr uint32_t kClaimBuckets = 32u;
 constexpr uint32_t kDeferHist = 32u;
 constexpr uint32_t kDeferFill = kDeferHist + kClaimBuckets;

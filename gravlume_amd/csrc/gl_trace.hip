@@ -637,6 +637,59 @@ __global__ void __launch_bounds__(64) k_refine_f64(const SceneDev* __restrict__ 
   }
 }
 
+// The exact f64 start of geodesic-batch request i.
+__device__ __forceinline__ void start_batch_f64(const SceneDev& s, const SceneT<double>& sd,
+                                                const FrameScene& f, uint32_t i,
+                                                RayT<double>& R) {
+  double y[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) y[k] = f.states[8 * size_t(i) + k];
+  const float dir = f.dirs[i] >= 0.0f ? 1.0f : -1.0f;
+  const uint32_t side = y[3] > 0.0 ? 2u : (y[3] < 0.0 ? 0u : 1u);
+  init_ray<double>(sd, R, y[1], y[2], y[3], 0.0, 0.0, 0.0, y[5], y[6], y[7], -y[4], dir, side);
+  R.h = double(dir) * double(s.h_init);
+  apply_start(R, start_test(s, y, true));  // an exact start: no rounding bits
+}
+
+// The f64 geodesic batch (gl_geodesic_batch_trace_f64): every request
+// traced in f64 throughout from its exact initial state at the caller's
+// tolerances, the reference batch's arithmetic (batch.rs:31-43 over
+// geodesic.rs:129-212). Lanes refill from a claim cursor as in the
+// finisher. Written as refined rays (GL_STATUS_REFINED).
+__global__ void __launch_bounds__(64) k_batch_f64(const SceneDev* __restrict__ scenes,
+                                                  const FrameScene* __restrict__ frame,
+                                                  MapView map, unsigned int* cursor, uint32_t n,
+                                                  double rtol, double atol) {
+  const FrameScene& f = frame[0];
+  const SceneDev& s = scenes[f.scene];
+  SceneT<double> sd = scene_f64(s);
+  sd.rtol = rtol;
+  sd.atol = atol;
+  if (f.out.turning_radius) sd.options |= kOptTurningRadius;
+  const double omega = double(s.omega);
+  RayT<double> R;
+  uint32_t it = 1;
+  unsigned int i = atomicAdd(cursor, 1u);
+  bool live = i < n;
+  if (live) start_batch_f64(s, sd, f, i, R);
+  while (live) {
+    if (R.term == kTermActive) {
+      step<double>(sd, R, omega, false, drift_now(it));
+      ++it;
+    }
+    if (R.term != kTermActive) {
+      write_ray(narrow_ray(R), f.out, i, s, map, batch_source(f.states + 8 * size_t(i)),
+                no_refine(), 0u, true);
+      i = atomicAdd(cursor, 1u);
+      live = i < n;
+      if (live) {
+        start_batch_f64(s, sd, f, i, R);
+        it = 1;
+      }
+    }
+  }
+}
+
 __global__ void k_queue_reset(QueueDev* q, unsigned long long total, unsigned int* defer_count) {
   if (blockIdx.x != 0) return;
   if (defer_count) clear_counters(defer_count);  // deferral, refine and ordering counters
@@ -1477,6 +1530,36 @@ gl_status gl_geodesic_batch_trace_noalloc(gl_trace_plan* p, uint32_t scene, cons
                      p->d_scenes, frame, states8, directions, uint32_t(n), map_view(p), df);
   GL_HIP(hipGetLastError());
   return launch_f64_kernels(p, frame, df, st);
+}
+
+gl_status gl_geodesic_batch_trace_f64(gl_trace_plan* p, uint32_t scene, const double* states8,
+                                      const float* directions, size_t n, double rtol,
+                                      double atol, const gl_ray_outputs* out, void* stream) {
+  if (!p || !out) return fail(GL_ERR_INVALID_ARGUMENT, "plan and outputs are required");
+  if (scene >= p->scenes.size()) return fail(GL_ERR_INVALID_ARGUMENT, "scene index out of range");
+  if (!(rtol >= 1e-15 && rtol <= 1e-3) || !(atol >= 1e-18 && atol <= 1e-3))
+    return fail(GL_ERR_INVALID_ARGUMENT, "f64 batch tolerances: rtol in [1e-15, 1e-3], atol in [1e-18, 1e-3]");
+  if (n == 0) return GL_OK;
+  if (!states8 || !directions) return fail(GL_ERR_INVALID_ARGUMENT, "states and directions are required");
+  if (n > 0xffffffffull) return fail(GL_ERR_CAPACITY, "more than 2^32 requests");
+  gl_status s = check_outputs(out, n);
+  if (s) return s;
+  hipStream_t st = (hipStream_t)stream;
+  s = ensure_workspace(p->ws, 1, false, false);
+  if (s) return s;
+  FrameScene f = make_frame(scene, 0, uint32_t(n), *out);
+  f.states = states8;
+  f.dirs = directions;
+  s = upload_frame(p, {f}, st);
+  if (s) return s;
+  hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, st, p->d_queue, 0ull, p->ws.counters);
+  mark(p, 0, st);
+  hipLaunchKernelGGL(k_batch_f64, dim3(std::max(1, p->cu_count * 4)), dim3(64), 0, st,
+                     p->d_scenes, p->d_frame, map_view(p), p->ws.counters + kBatchCursor,
+                     uint32_t(n), rtol, atol);
+  GL_HIP(hipGetLastError());
+  mark(p, 1, st);
+  return GL_OK;
 }
 
 gl_status gl_trace_plan_queue_stats(gl_trace_plan* p, gl_queue_stats* stats, void* stream) {

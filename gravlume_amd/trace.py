@@ -379,7 +379,7 @@ class GeodesicBatch:
 
     def trace_ordered(self, states: np.ndarray, directions: np.ndarray, stream=None,
                       persistent: bool = False, config: dict | None = None,
-                      no_alloc: bool = False) -> dict:
+                      no_alloc: bool = False, f64_tolerance: tuple | None = None) -> dict:
         """Host arrays in, decoded host outputs back (synchronizes)."""
         states = np.ascontiguousarray(states, dtype=np.float64).reshape(-1, 8)
         n = states.shape[0]
@@ -392,21 +392,31 @@ class GeodesicBatch:
             return target.to_host()
         d_states = torch.from_numpy(states).cuda()
         d_dirs = torch.from_numpy(dirs).cuda()
-        self.encode(d_states, d_dirs, target, stream, persistent, config, no_alloc)
+        self.encode(d_states, d_dirs, target, stream, persistent, config, no_alloc, f64_tolerance)
         torch.cuda.synchronize()
         return target.to_host()
 
     def encode(self, d_states: torch.Tensor, d_dirs: torch.Tensor, target: TraceTarget,
                stream=None, persistent: bool = False, config: dict | None = None,
-               no_alloc: bool = False) -> None:
+               no_alloc: bool = False, f64_tolerance: tuple | None = None) -> None:
         """Device arrays in (float64 (n, 8) states, float32 (n,) directions),
         outputs into `target`; enqueued on `stream`, no synchronization.
         ``no_alloc`` selects gl_geodesic_batch_trace_noalloc (no device
-        allocation; at most NO_ALLOC_MAX_REQUESTS requests per call)."""
+        allocation; at most NO_ALLOC_MAX_REQUESTS requests per call);
+        ``f64_tolerance = (rtol, atol)`` gl_geodesic_batch_trace_f64 (every
+        request in f64 at that tolerance)."""
         n = d_states.shape[0]
         out = target.c_struct()
         lib = self.pipeline._lib
-        if no_alloc:
+        if f64_tolerance is not None:
+            if persistent or no_alloc:
+                raise ValueError("the f64 batch has its own kernel")
+            rtol, atol = f64_tolerance
+            _capi.check(lib.gl_geodesic_batch_trace_f64(
+                self.pipeline.handle, self.scene, C.c_void_p(d_states.data_ptr()),
+                C.c_void_p(d_dirs.data_ptr()), n, float(rtol), float(atol), C.byref(out),
+                _stream_handle(stream)))
+        elif no_alloc:
             if persistent:
                 raise ValueError("the allocation-free batch runs the plain batch kernel")
             _capi.check(lib.gl_geodesic_batch_trace_noalloc(

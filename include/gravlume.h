@@ -375,6 +375,19 @@ gl_status gl_geodesic_batch_trace_persistent(gl_trace_plan* plan, uint32_t scene
                                              size_t n, const gl_persistent_config* config,
                                              const gl_ray_outputs* out, void* stream);
 
+/* The geodesic batch in f64 throughout (GeodesicBatch::trace_ordered,
+ * batch.rs:31-43, with the reference tracer's arithmetic, geodesic.rs:129-212):
+ * every request is traced by f64 DP5(4) from its exact initial state at the
+ * given tolerances (rtol in [1e-15, 1e-3], atol in [1e-18, 1e-3]; the scene
+ * supplies the step policy and the events), with the near-critical test of
+ * an exact start and f64 drifts. Results in request order, marked
+ * GL_STATUS_REFINED. Device inputs and outputs as for
+ * gl_geodesic_batch_trace. */
+gl_status gl_geodesic_batch_trace_f64(gl_trace_plan* plan, uint32_t scene,
+                                      const double* states8, const float* directions, size_t n,
+                                      double rtol, double atol, const gl_ray_outputs* out,
+                                      void* stream);
+
 /* gl_geodesic_batch_trace without any device allocation, for callers that
  * cannot allocate (code under HIP graph capture, tight serving loops,
  * harnesses that forbid hipMalloc). No counterpart in the reference. The

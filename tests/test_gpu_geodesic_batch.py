@@ -18,7 +18,14 @@ fixtures and the oracle's failure classes (needs an MI355X).
   the budget, so the GPU protocol reports the guard as Uncertain with the
   class reached kept, geodesic_integration.wgsl:324-330) and a non-finite
   momentum (numerical failure), each with the oracle's class;
-* GeodesicBatch::new takes 1..256 threads (batch.rs:19-29).
+* GeodesicBatch::new takes 1..256 threads (batch.rs:19-29);
+* the f64 batch (gl_geodesic_batch_trace_f64): the three v1 fixtures within
+  their own tolerance plus the binary32 rounding of the output planes, and
+  512 near-critical requests of the a = 0.998 batch against the oracle at
+  the same tolerance (rtol 1e-12): every definite class and event id equal
+  on rays the oracle does not flag, windings and turnings equal where the
+  branch is settled, travel time, turning radius and azimuth advance
+  within 1e-6 relative.
 """
 import json
 import math
@@ -162,3 +169,64 @@ def test_batch_thread_count_is_checked(gl):
         with pytest.raises(gl.trace.GeodesicBatchError):
             gl.GeodesicBatch(pipe, 0, threads=bad)
     assert gl.GeodesicBatch(pipe, 0, threads=256).threads == 256
+
+
+@pytest.mark.parametrize("name", list(_fixtures()))
+def test_f64_batch_fixtures(gl, name):
+    from gravlume_amd import scene as S
+    fx = _fixtures()[name]
+    sc = _scene(S, fx)
+    pipe = gl.TracePipeline([sc])
+    out = gl.GeodesicBatch(pipe, 0).trace_ordered(_state(fx), np.array([float(fx["affine_direction"])]),
+                                                  f64_tolerance=(1e-13, 1e-15))
+    exp, tol = fx["expected"], fx["tolerance"]
+    assert out["termination"][0] == {"escape": 2, "horizon": 1}[exp["termination"]]
+    assert out["refined"][0]
+    adv_ref = float(exp["azimuth_advance"])
+    assert abs(float(out["azimuth_advance"][0]) - adv_ref) <= (float(tol["azimuth_advance"]) +
+                                                               2 * F32_ULP * abs(adv_ref))
+    if "turning_radius" in exp:
+        rt_ref = float(exp["turning_radius"])
+        assert out["radial_turnings"][0] == 1
+        assert abs(float(out["turning_radius"][0]) - rt_ref) <= (float(tol["turning_radius"]) +
+                                                                 2 * F32_ULP * rt_ref)
+    else:
+        assert abs(out["event_residual"][0]) * 2.0 <= float(tol["event_radius"]) + 2 * F32_ULP * 2.0
+
+
+def test_f64_batch_matches_the_oracle(gl, oracle):
+    import math
+    from gravlume_amd import critical as Cr, scene as S
+    sc = S.near_critical_scene()
+    states, delta, *_ = Cr.near_critical_states(sc.spin, sc.charge, sc.observer_radius,
+                                                math.radians(sc.inclination_deg), 512,
+                                                seed=20261212, chart=sc.chart)
+    dirs = -np.ones(len(states))
+    pipe = gl.TracePipeline([sc])
+    h = gl.GeodesicBatch(pipe, 0).trace_ordered(states, dirs, f64_tolerance=(1e-12, 1e-14))
+    st = oracle.spacetime(1.0, sc.spin, sc.charge, float(sc.chart))
+    ev = oracle.events(sc.escape_radius, None, horizon_offset=sc.horizon_offset)
+    A = oracle.outcomes_to_arrays(oracle.trace_ordered(st, oracle.config_policy(1e-12, 1e-14), ev,
+                                                       states, dirs, threads=16))
+    F = oracle.outcomes_to_arrays(oracle.trace_ordered(
+        st, oracle.config_policy(1e-12 / 16, 1e-14 / 16), ev, states, dirs, threads=16))
+    ok = (A["termination"] == F["termination"]) & (A["candidates"] == F["candidates"])
+    t = h["termination"]
+    assert np.all(t > 0) and np.all(h["refined"])
+    definite = ok & (t != 6)
+    print("f64 batch: flagged", int((~ok).sum()), "uncertain", int((ok & (t == 6)).sum()),
+          "branch unsettled", int((definite & h["branch_unsettled"]).sum()))
+    assert np.mean(t[ok] == 6) <= 0.01
+    assert np.array_equal(t[definite], A["termination"][definite])
+    assert np.array_equal(h["candidates"][definite], A["candidates"][definite])
+    settled = definite & ~h["branch_unsettled"]
+    assert settled.sum() >= 0.9 * len(states)
+    for key in ("azimuth_winding", "radial_turnings", "equatorial_crossings", "polar_side"):
+        assert np.array_equal(h[key][settled], A[key][settled]), key
+    tt = np.abs(h["travel_time"][settled] / A["travel_time"][settled] - 1.0)
+    assert tt.max() <= 1e-6, tt.max()
+    adv = np.abs(h["azimuth_advance"][settled] - A["azimuth_advance"][settled])
+    assert np.all(adv <= 1e-6 * np.maximum(1.0, np.abs(A["azimuth_advance"][settled]))), adv.max()
+    turned = settled & (A["radial_turnings"] > 0)
+    rt = np.abs(h["turning_radius"][turned] / A["turning_radius"][turned] - 1.0)
+    assert rt.max() <= 1e-6, rt.max()
