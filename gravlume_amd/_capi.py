@@ -56,6 +56,7 @@ STATUS_PROVISIONAL_SHIFT = 16
 STATUS_NEAR_CRITICAL = 1 << 20
 STATUS_REFINED = 1 << 21
 STATUS_BRANCH_UNSETTLED = 1 << 22
+STATUS_NO_ADJOINT = 1 << 23
 
 EMISSION_BOLOMETRIC = 0
 EMISSION_BLACKBODY = 1

@@ -75,8 +75,11 @@ __global__ void __launch_bounds__(64) k_vjp_forward(const SceneDev* __restrict__
   const size_t rays = vb.rays;
   RayT<float> R;
   init_image_ray(s, sc, px, py, R);
-  bool overflow = false;
+  bool overflow = false, island = false;
   for (uint32_t it = 1; R.term == kTermActive; ++it) {
+    // where the trace would hand the ray to its f64 finisher, this forward
+    // goes on in f32: such a ray is not the trace's, and gets no gradient
+    island = island || wants_defer(sc, R);
     const float pre[6] = {R.x - R.cx, R.y - R.cy, R.z - R.cz, R.px, R.py, R.pz};
     const float h = R.h;
     const uint32_t acc = R.accepted;
@@ -98,8 +101,10 @@ __global__ void __launch_bounds__(64) k_vjp_forward(const SceneDev* __restrict__
     }
   }
   vb.E[idx] = R.E;
-  vb.n[idx] = (R.term == kTermSurface && !overflow) ? R.accepted : 0u;
+  vb.n[idx] = (R.term == kTermSurface && !overflow && !island) ? R.accepted : 0u;
   write_ray(R, out, idx, s, map, pixel_source(px, py), no_refine(), 0u);
+  if (R.term == kTermSurface && (overflow || island) && out.status)
+    out.status[idx] |= GL_STATUS_NO_ADJOINT;
 }
 
 // Adjoint of the terminal (surface-crossing) step, seeded by dL/d(r, phi_s,

@@ -90,6 +90,7 @@ class TraceTarget:
             "near_critical": (st & _capi.STATUS_NEAR_CRITICAL) != 0,
             "refined": (st & _capi.STATUS_REFINED) != 0,
             "branch_unsettled": (st & _capi.STATUS_BRANCH_UNSETTLED) != 0,
+            "no_adjoint": (st & _capi.STATUS_NO_ADJOINT) != 0,
             "event_residual": self.event_residual.cpu().numpy(),
             "azimuth_advance": self.azimuth_advance.cpu().numpy(),
             "steps": self.steps.cpu().numpy().view(np.uint32).astype(np.int64),

@@ -56,7 +56,8 @@ enum {
  *              curve (the radial-potential margin of its constants of
  *              motion) lies below the trace's own error budget
  *   bit  21    GL_STATUS_REFINED: the ray was traced again in f64 at the
- *              scene's refine tolerance (GL_OPT_NEAR_CRITICAL_REFINE)
+ *              scene's refine tolerance (GL_OPT_NEAR_CRITICAL_REFINE), or in
+ *              f64 throughout by gl_geodesic_batch_trace_f64
  *   bit  22    GL_STATUS_BRANCH_UNSETTLED: the margin lies below
  *              branch_factor times the error budget, so the branch key
  *              (turnings, crossings, winding), the travel time and the
@@ -64,6 +65,11 @@ enum {
  *              critical orbit, are not settled; the class may still be
  *              definite (the margin lies above near_critical_factor times
  *              the budget)
+ *   bit  23    GL_STATUS_NO_ADJOINT (gl_vjp_forward only): a surface ray the
+ *              adjoint gives no gradient, because its trace took more
+ *              accepted steps than the adjoint records or entered the f64
+ *              near-horizon island, which the adjoint forward does not use
+ *              (its outputs there differ from the trace's)
  * A ray is GL_TERM_UNCERTAIN when its terminal events tie
  * (GL_FLAG_TIED_EVENTS), a radial turning lies inside the tie window of the
  * terminal (GL_FLAG_TURNING_ORDER), an invariant drift exceeds the scene
@@ -73,6 +79,7 @@ enum {
 #define GL_STATUS_NEAR_CRITICAL (1u << 20)
 #define GL_STATUS_REFINED (1u << 21)
 #define GL_STATUS_BRANCH_UNSETTLED (1u << 22)
+#define GL_STATUS_NO_ADJOINT (1u << 23)
 
 /* Event-candidate bits (trace_protocol.wgsl:95-98, geodesic.rs:616-634). */
 enum {
@@ -439,7 +446,9 @@ gl_status gl_frame_graph_destroy(gl_frame_graph* graph);
  * The forward writes the trace outputs to `out` and records the accepted
  * step sizes (up to min(max_accepted_steps, 1024) per ray; longer rays get
  * no gradient), a checkpoint every `checkpoint_interval` accepted steps and
- * the terminal state; no f64 near-horizon deferral is used. The backward
+ * the terminal state; no f64 near-horizon deferral is used (surface rays
+ * that entered the island get no gradient either; both carry
+ * GL_STATUS_NO_ADJOINT in `out`'s status plane). The backward
  * synchronizes `stream`. With plan timing enabled, both kernels are timed
  * like a trace launch (gl_trace_plan_kernel_ms). */
 typedef struct gl_vjp_state gl_vjp_state;

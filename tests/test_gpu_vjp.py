@@ -138,3 +138,33 @@ def test_autograd_function_contract(gl):
     loss.backward()
     assert params.grad is not None and torch.all(torch.isfinite(params.grad))
     assert torch.any(params.grad != 0)
+
+
+def test_vjp_forward_is_the_trace_where_it_differentiates(gl):
+    """The adjoint's recording forward gives the plain trace's outputs bit for
+    bit on every surface pixel it differentiates; the surface pixels it
+    cannot differentiate (more accepted steps than it records, or a visit
+    to the f64 near-horizon island, which it does not take) carry
+    GL_STATUS_NO_ADJOINT and receive no direction gradient."""
+    import torch
+    from gravlume_amd import scene as S
+    sc = S.adjoint_scene(width=256, height=256)
+    pipe = gl.TracePipeline([sc])
+    plain = pipe.create_target(0)
+    pipe.encode_batch(plain)
+    vjp = gl.VjpTrace(pipe, 0, checkpoint_interval=32)
+    torch.cuda.synchronize()
+    hp, hv = plain.to_host(), vjp.target.to_host()
+    srf = (hv["termination"] == 7) & ~hv["no_adjoint"]
+    assert srf.sum() > 10000
+    for k in hp:
+        if k == "no_adjoint":
+            continue
+        assert np.array_equal(np.asarray(hp[k])[srf], np.asarray(hv[k])[srf]), k
+    seed = torch.zeros(sc.rays, dtype=torch.float32, device="cuda")
+    seed[torch.from_numpy(np.flatnonzero(hv["termination"] == 7)).cuda()] = 1.0
+    gp, gd = vjp.backward(seed, direction=True)  # gd: (2, rays)
+    nog = torch.from_numpy(hv["no_adjoint"]).cuda()
+    assert torch.all(gd[:, nog] == 0)
+    assert torch.any(gd[:, torch.from_numpy(srf).cuda()] != 0)
+    print("surface pixels without an adjoint:", int(hv["no_adjoint"].sum()))
