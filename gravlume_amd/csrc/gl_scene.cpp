@@ -305,6 +305,7 @@ int prepare_scene(const gl_scene_desc& d, SceneDev* out, SceneHost* host, std::s
   V4 arrival;
   for (int i = 0; i < 4; ++i) arrival.v[i] = -sight.v[i];
   V4 up;
+  int up_axis = 0;
   V4 up_seed = {{0.0, d.up_hint[0], d.up_hint[1], d.up_hint[2]}};
   if (!orthonormalize(m, u, &sight, 1, up_seed, &up)) {
     // Fallback: the coordinate axis whose normalized remainder has the
@@ -320,6 +321,7 @@ int prepare_scene(const gl_scene_desc& d, SceneDev* out, SceneHost* host, std::s
       if (score >= best * (1.0 - 1.0e-9)) {  // ties go to the later axis
         best = score;
         up = c;
+        up_axis = a;
         found = true;
       }
     }
@@ -474,6 +476,7 @@ int prepare_scene(const gl_scene_desc& d, SceneDev* out, SceneHost* host, std::s
       host->arrival[i] = arrival.v[i];
     }
     host->tan_half_fov = thf;
+    host->up_axis = up_axis;
     host->desc = d;
   }
   return GL_OK;

@@ -16,6 +16,7 @@ struct SceneHost {
   double g_tt;
   double u[4], right[4], up[4], arrival[4];
   double tan_half_fov;
+  int up_axis;  // 0: the image-up hint; 1-3: the coordinate axis of the fallback
 };
 
 // Validates one scene and fills its device record; returns a gl_status and

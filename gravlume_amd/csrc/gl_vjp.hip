@@ -395,6 +395,13 @@ gl_status observer_tangents(const SceneHost& hs, double inclination, VjpTangents
     if (k == 2) tp += step, tm -= step;
     if (observer_at(d, r, phi, ap, qp, tp, &hp, &err) || observer_at(d, r, phi, am, qm, tm, &hm, &err))
       return fail(GL_ERR_DOMAIN, "perturbed scene for the observer tangents: " + err);
+    // The right leg is the one direction left by (u, up, arrival), whichever
+    // axis seeds it; the image-up fallback is not: a different fallback axis
+    // on the two sides of the difference would make it meaningless
+    if (hp.up_axis != hs.up_axis || hm.up_axis != hs.up_axis)
+      return fail(GL_ERR_DOMAIN,
+                  "the observer's image-up axis changes within the tangent step; choose an up "
+                  "hint that is not along the line of sight");
     const double inv = 1.0 / (2.0 * step);
     for (int c = 0; c < 3; ++c)
       t->obs[k][c] = float((hp.desc.observer_txyz[c + 1] - hm.desc.observer_txyz[c + 1]) * inv);
