@@ -447,8 +447,9 @@ struct Verdict {
 // start test its trace carried (r_m, |mu| and the start bits, from the exact
 // f64 initial state): a ray that came within 1.25 r_m (its smallest turning
 // radius, or its terminal radius, r_near) is measured against its error
-// budget, the largest drift it sampled at r in [r_m, 2 r_m] and, for an f32
-// trace, the rounding of its initial constants (the start bits): with
+// budget, the largest drift it sampled at r from max(7/8 r_m, (r_m + r_+) / 2)
+// to 2 r_m and, for an f32 trace, the rounding of its initial constants (the
+// start bits): with
 // |mu| <= near_critical_factor * budget the capture/scatter decision is
 // unsettled (GL_STATUS_NEAR_CRITICAL), with |mu| <= branch_factor * budget
 // the branch key and the time and azimuth spent near r_m are

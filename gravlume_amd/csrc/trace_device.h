@@ -303,8 +303,9 @@ struct RayT {
   // the turning step (enough for the near-critical test's proximity check).
   T r_turn;
   // The near-critical test (near_critical.h): d_crit, the largest relative
-  // drift sampled at r in [r_m, 2 r_m], where the capture/scatter decision
-  // is taken; r_m and the margin ride in the high bits of `armed`.
+  // drift sampled around r_m (max(7/8 r_m, (r_m + r_+) / 2) <= r <= 2 r_m),
+  // where the capture/scatter decision is taken; r_m and the margin ride in
+  // the high bits of `armed`.
   T d_crit;
   uint32_t accepted, rejected, consec;
   uint32_t turnings, crossings;
@@ -755,11 +756,19 @@ GL_HD void update_drift(const SceneT<T>& sc, RayT<T>& R, const Geom<T>& g, const
   R.d_lz = gl_max<T>(R.d_lz, dl);
   R.d_q = gl_max<T>(R.d_q, dq);
   // the near-critical budget counts the drift where the capture/scatter
-  // decision is taken, r in [r_m, 2 r_m]: beyond, the invariants' own
-  // evaluation (terms of size r^2 p^2 cancelling to Carter's constant) is
-  // the larger error, and below r_m a captured ray is already decided
+  // decision is taken, around r_m: up to 2 r_m (beyond, the invariants' own
+  // evaluation, terms of size r^2 p^2 cancelling to Carter's constant, is
+  // the larger error) and down to 7/8 r_m, for a ray near the critical curve
+  // orbits at r ~ r_m on either side of it (its constants differ from the
+  // exact ray's by the drift, and the carried r_m is rounded up by up to
+  // 2^-8; a band starting at r_m missed the orbits of rays that circled ~30
+  // times and left with the wrong class), but not below halfway to the
+  // horizon: there the ray is decided, and in a chart singular on the
+  // horizon its momentum, and with it the evaluation error of Carter's
+  // constant, grows without bound.
   const T r_m = T(decode_start_radius(R.armed));
-  if (g.r >= r_m && g.r <= T(2) * r_m)
+  const T r_lo = gl_max<T>(T(0.875) * r_m, T(0.5) * (r_m + sc.r_capture));
+  if (g.r >= r_lo && g.r <= T(2) * r_m)
     R.d_crit = gl_max<T>(R.d_crit, gl_max<T>(nr, gl_max<T>(dl, dq)));
   if (!(R.d_null <= sc.drift_limit && R.d_lz <= sc.drift_limit && R.d_q <= sc.drift_limit))
     R.flags |= kFlagDrift;

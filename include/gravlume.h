@@ -186,7 +186,8 @@ typedef struct gl_scene_desc {
    * where its radial potential R(r) is smallest over [r_+, r_start] is
    * near-critical when |R(r_m)| / (sum of the magnitudes of R's two terms at
    * r_m) <= near_critical_factor * (the trace's error budget: the largest of
-   * its relative null, L_z and Carter drifts sampled at r in [r_m, 2 r_m],
+   * its relative null, L_z and Carter drifts sampled at r from
+   * max(7/8 r_m, (r_m + r_+) / 2) to 2 r_m,
    * and for an f32 trace the f32 rounding of its initial constants). 0
    * selects 16; negative disables the test (and the branch test). Scenes
    * without a horizon are never near-critical. */
