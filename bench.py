@@ -49,6 +49,9 @@ def parse():
     ap.add_argument("--autotune", action="store_true",
                     help="pick the persistent launch configuration with the autotuner instead")
     ap.add_argument("--copy-calibration-mib", type=int, default=0)
+    ap.add_argument("--only", choices=["near_critical"], default=None,
+                    help="run one workload's leg alone and print its JSON (for profiling); the "
+                         "default runs the full bench line")
     return ap.parse_args()
 
 
@@ -545,6 +548,9 @@ def main():
     from gravlume_amd import scene as S
     from gravlume_amd.autotune import autotune
 
+    if args.only == "near_critical":
+        print(json.dumps({"near_critical": near_critical_leg(g, S, torch)}), flush=True)
+        return
     scene = S.single_frame_1080p()
     pipe = g.TracePipeline([scene])
     pipe.enable_timing(True)
