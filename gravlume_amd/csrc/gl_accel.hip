@@ -219,13 +219,8 @@ __global__ void __launch_bounds__(64, 2) k_sky_fused(
       RayT<float> R;
       init_image_ray(s, sc, px, py, R);
       bool deferred = false;
-      for (uint32_t it = 1; R.term == kTermActive; ++it) {
-        if (wants_defer(sc, R) && try_defer(df, R, 0u, idx)) {
-          deferred = true;
-          break;
-        }
-        step<float>(sc, R, s.omega, reproject_now(sc, it), drift_now(it));
-      }
+      uint32_t it = 1;
+      trace_f32(sc, R, s.omega, df, 0u, idx, it, deferred);
       // a deferred or refined node is finished in f64 later: not a stencil node
       if (!deferred)
         deferred = !write_ray(R, fsc.out, idx, s, map, pixel_source(px, py), refine_sink(df), 0u);

@@ -667,4 +667,33 @@ __device__ __forceinline__ bool try_defer(const DeferBuf& df, const RayT<float>&
   return true;
 }
 
+// The f32 stepping loop of one ray from attempt `it`: DP5 attempts until
+// the ray ends, or until it enters the f64 island and its record goes to the
+// finisher (deferred = true). A deferral the buffer cannot take (it is full
+// for the rest of the launch) leaves the ray in f32. The loop exits when no
+// lane of the wavefront has a ray left to step, so its exit is
+// wavefront-uniform: a lane whose ray is done idles, masked, until then. A
+// loop that lanes leave at different trips makes the compiler copy every
+// loop-carried value of the ray state on each trip (its value at the lane's
+// exit must survive the later trips of the other lanes).
+__device__ __forceinline__ void trace_f32(const SceneT<float>& sc, RayT<float>& R, float omega,
+                                          const DeferBuf& df, uint32_t fs, uint32_t idx,
+                                          uint32_t& it, bool& deferred) {
+  bool may_defer = (sc.options & kOptF64Horizon) != 0u;
+  for (;;) {
+    const bool live = !deferred && R.term == kTermActive;
+    if (!__any(live)) break;
+    if (live) {
+      if (!(may_defer && R.r < sc.r_defer)) {
+        step<float>(sc, R, omega, reproject_now(sc, it), drift_now(it));
+        ++it;
+      } else if (try_defer(df, R, fs, idx)) {
+        deferred = true;
+      } else {
+        may_defer = false;
+      }
+    }
+  }
+}
+
 }  // namespace gl

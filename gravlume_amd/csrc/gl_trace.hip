@@ -47,11 +47,10 @@ __global__ void __launch_bounds__(64) k_plain_image(const SceneDev* __restrict__
   uint32_t idx = py * s.width + px;
   RayT<float> R;
   init_image_ray(s, sc, px, py, R);
-  for (uint32_t it = 1; R.term == kTermActive; ++it) {
-    if (wants_defer(sc, R) && try_defer(df, R, 0u, idx)) return;
-    step<float>(sc, R, s.omega, reproject_now(sc, it), drift_now(it));
-  }
-  write_ray(R, fsc.out, idx, s, map, pixel_source(px, py), refine_sink(df), 0u);
+  uint32_t it = 1;
+  bool deferred = false;
+  trace_f32(sc, R, s.omega, df, 0u, idx, it, deferred);
+  if (!deferred) write_ray(R, fsc.out, idx, s, map, pixel_source(px, py), refine_sink(df), 0u);
 }
 
 __global__ void __launch_bounds__(64) k_geodesic_batch(const SceneDev* __restrict__ scenes,
@@ -67,11 +66,11 @@ __global__ void __launch_bounds__(64) k_geodesic_batch(const SceneDev* __restric
   if (i >= n) return;
   RayT<float> R;
   init_state_ray(s, sc, states + 8 * size_t(i), dirs[i] >= 0.0f ? 1.0f : -1.0f, R);
-  for (uint32_t it = 1; R.term == kTermActive; ++it) {
-    if (wants_defer(sc, R) && try_defer(df, R, 0u, i)) return;
-    step<float>(sc, R, s.omega, reproject_now(sc, it), drift_now(it));
-  }
-  write_ray(R, fsc.out, i, s, map, batch_source(states + 8 * size_t(i)), refine_sink(df), 0u);
+  uint32_t it = 1;
+  bool deferred = false;
+  trace_f32(sc, R, s.omega, df, 0u, i, it, deferred);
+  if (!deferred)
+    write_ray(R, fsc.out, i, s, map, batch_source(states + 8 * size_t(i)), refine_sink(df), 0u);
 }
 
 __device__ __forceinline__ uint32_t find_frame_scene(const FrameScene* frame, uint32_t n,
@@ -251,13 +250,8 @@ __global__ void __launch_bounds__(256, kWaves) k_persistent_tiles(
     }
     if (have) {
       bool deferred = false;
-      for (it = 1; R.term == kTermActive; ++it) {
-        if (wants_defer(sc, R) && try_defer(df, R, fs, idx)) {
-          deferred = true;
-          break;
-        }
-        step<float>(sc, R, s.omega, reproject_now(sc, it), drift_now(it));
-      }
+      it = 1;
+      trace_f32(sc, R, s.omega, df, fs, idx, it, deferred);
       lane_attempts += it - 1;
       if (!deferred) write_ray(R, f.out, idx, s, map, frame_source(f, s, idx), refine_sink(df), fs);
     }

@@ -783,14 +783,12 @@ GL_HD void update_drift(const SceneT<T>& sc, RayT<T>& R, const Geom<T>& g, const
 // Chords span at most a fraction of a turn (|h| <= h_max_per_r r), where
 // both tests agree.
 template <class T>
-GL_HD void update_winding(RayT<T>& R, T x0, T y0, T x1, T y1) {
+GL_HD int32_t winding_step(T x0, T y0, T x1, T y1) {
   const bool up0 = y0 >= T(0), up1 = y1 >= T(0);
-  if (up0 == up1) return;
   // x at the crossing, scaled by (y0 - y1) to avoid a division
-  T xc = x0 * (y0 - y1) + (x1 - x0) * y0;
-  bool neg = (y0 - y1) > T(0) ? xc < T(0) : xc > T(0);
-  if (!neg) return;
-  R.winding += up0 ? 1 : -1;
+  const T xc = x0 * (y0 - y1) + (x1 - x0) * y0;
+  const bool neg = (y0 - y1) > T(0) ? xc < T(0) : xc > T(0);
+  return (up0 != up1 && neg) ? (up0 ? 1 : -1) : 0;
 }
 
 // Source anchor (r, phi_s) and frequency ratio g of the prograde Keplerian
@@ -835,13 +833,19 @@ GL_HD bool surface_observable(U mass, U spin, U charge2, U chart, U omega, U x, 
 // detected and localized, and the branch key, drifts and arming are
 // committed; on termination R.term becomes nonzero and (x, p) hold the
 // localized terminal state.
+//
+// The attempt is straight-line code up to its outcome, and the ray state is
+// committed with selects: an early return (step exhaustion, a non-finite
+// stage, a rejection) would make the compiler merge the old and the new
+// state at the join, and it does so through a third set of registers, with
+// a copy of every state value on both paths. Only rare work sits behind
+// branches: the event roots, the terminal localization, the drift sample and
+// the re-projection. Step exhaustion is decided before the attempt, whose
+// arithmetic is then discarded.
 template <class T>
 GL_HD void step(const SceneT<T>& sc, RayT<T>& R, T omega, bool reproject = false,
                  bool drift_now = true) {
-  if (R.accepted >= sc.max_accepted) {
-    R.term = kTermStepExhaustion;
-    return;
-  }
+  const bool exhausted = R.accepted >= sc.max_accepted;
   const T h = R.h;
   T K[7][7];
 #pragma unroll
@@ -929,32 +933,24 @@ GL_HD void step(const SceneT<T>& sc, RayT<T>& R, T omega, bool reproject = false
   // the new state through its weights or through the later stages)
   const T probe = xn[0] + xn[1] + xn[2] + pn[0] + pn[1] + pn[2] + dtinc + err[0] + err[1] +
                   err[2] + err[3] + err[4] + err[5] + err[6];
-  if (!ok || !(en == en) || en > T(3.0e38) || !finite_val<T>(probe)) {
-    // A non-finite stage is a typed numerical failure (integrator.rs:170-173).
-    R.term = kTermFailure;
-    R.flags |= kFlagNonFinite;
-    return;
-  }
+  const bool fail = !ok || !(en == en) || en > T(3.0e38) || !finite_val<T>(probe);
+  const bool live = !exhausted && !fail;
+  const bool reject = live && en > T(1);
+  const bool accept = live && !(en > T(1));
   const T hmag = gl_abs<T>(h);
-  if (en > T(1)) {
-    R.rejected += 1;
-    R.consec += 1;
-    if (R.consec >= sc.max_rejects) {
-      R.term = kTermRejectExhaustion;
-      return;
-    }
-    if (hmag <= sc.h_min) {
-      R.term = kTermFailure;
-      return;
-    }
-    T f = gl_clamp<T>(sc.safety * pow_m02<T>(en), sc.min_factor, T(1));
-    R.h = T(R.dir) * gl_clamp<T>(hmag * f, sc.h_min, step_cap<T>(sc, R.r));
-    return;
-  }
-  R.accepted += 1;
-  const bool after_reject = R.consec != 0;
-  R.consec = 0;
   const T dir = T(R.dir);
+  // The controller factor 0.9 E^-1/5 (policy.rs:160-173), shared by both
+  // outcomes.
+  const T pw = sc.safety * pow_m02<T>(en);
+
+  // Rejection (policy.rs:137-173): the reject limit, then the minimum step.
+  const uint32_t consec_rej = R.consec + 1u;
+  const bool rej_exhausted = reject && consec_rej >= sc.max_rejects;
+  const bool rej_failed = reject && !rej_exhausted && hmag <= sc.h_min;
+  const T f_rej = gl_clamp<T>(pw, sc.min_factor, T(1));
+  const T h_rej = dir * gl_clamp<T>(hmag * f_rej, sc.h_min, step_cap<T>(sc, R.r));
+
+  const bool after_reject = R.consec != 0;
   const T r0 = R.r, r1 = g.r;
   const T rd1 = radial_rate<T>(g, d);
   const T z0 = y0[2], z1 = xn[2];
@@ -962,19 +958,20 @@ GL_HD void step(const SceneT<T>& sc, RayT<T>& R, T omega, bool reproject = false
   const bool turning = (v0 < T(0) && v1 >= T(0)) || (v0 > T(0) && v1 <= T(0));
   const bool zcross = (z0 < T(0) && z1 >= T(0)) || (z0 > T(0) && z1 <= T(0));
 
-  // Terminal event candidates (fraction > 1 means none).
+  // Terminal event candidates of an accepted attempt (fraction > 1 means
+  // none).
   T fr[4] = {T(2), T(2), T(2), T(2)};
   // the measure r^4 + a^2 z^2 falls to the guard only where r^4 <= guard
-  if ((R.armed & kEvSing) && (r1 * r1) * (r1 * r1) <= sc.guard) {
+  if (accept && (R.armed & kEvSing) && (r1 * r1) * (r1 * r1) <= sc.guard) {
     T m0 = singularity_measure<T>(r0, sc.spin2, z0) - sc.guard;
     T m1 = singularity_measure<T>(r1, sc.spin2, z1) - sc.guard;
     if (m0 > T(0) && m1 <= T(0)) fr[0] = gl_clamp<T>(m0 * gl_rcp<T>(m0 - m1), T(0), T(1));
   }
-  if ((R.armed & kEvHor) && r0 > sc.r_capture && r1 <= sc.r_capture)
+  if (accept && (R.armed & kEvHor) && r0 > sc.r_capture && r1 <= sc.r_capture)
     fr[1] = radius_root<T>(sc, K, h, y0[0], y0[1], y0[2], r0 - sc.r_capture, r1 - sc.r_capture,
                            sc.r_capture);
   T thz = T(2);
-  if (zcross && (R.armed & kEvSurf)) {
+  if (accept && zcross && (R.armed & kEvSurf)) {
     thz = z_root<T, T>(K, h, z0, z1);
     T xs, ysv, zs;
     dense_position<T>(K, h, y0[0], y0[1], y0[2], thz, xs, ysv, zs);
@@ -984,7 +981,7 @@ GL_HD void step(const SceneT<T>& sc, RayT<T>& R, T omega, bool reproject = false
     const T r2s = xs * xs + ysv * ysv - sc.spin2;
     if (r2s >= sc.disk_in * sc.disk_in && r2s <= sc.disk_out * sc.disk_out) fr[2] = thz;
   }
-  if ((R.armed & kEvEsc) && r0 < sc.r_escape && r1 >= sc.r_escape)
+  if (accept && (R.armed & kEvEsc) && r0 < sc.r_escape && r1 >= sc.r_escape)
     fr[3] = radius_root<T>(sc, K, h, y0[0], y0[1], y0[2], r0 - sc.r_escape, r1 - sc.r_escape,
                            sc.r_escape);
   int kind = 0;
@@ -996,7 +993,46 @@ GL_HD void step(const SceneT<T>& sc, RayT<T>& R, T omega, bool reproject = false
       kind = i;
     }
 
-  if (best <= T(1)) {
+  const bool terminal = best <= T(1);  // only an accepted attempt has candidates
+  const bool commit = accept && !terminal;
+
+  // Counters and outcome of the attempt.
+  R.accepted += accept ? 1u : 0u;
+  R.rejected += reject ? 1u : 0u;
+  R.consec = reject ? consec_rej : (accept ? 0u : R.consec);
+  uint32_t term = R.term;
+  if (exhausted) term = kTermStepExhaustion;
+  if (!exhausted && fail) term = kTermFailure;
+  if (rej_exhausted) term = kTermRejectExhaustion;
+  if (rej_failed) term = kTermFailure;
+  R.term = term;
+  // A non-finite stage is a typed numerical failure (integrator.rs:170-173).
+  R.flags |= (!exhausted && fail) ? kFlagNonFinite : 0u;
+
+  // What a non-terminal accepted attempt reads of the new state, before the
+  // state is committed: the drift sample, the radial turning, the crossing
+  // and the winding.
+  if (commit && (reproject || drift_now))
+    update_drift<T>(sc, R, g, d, xn[0], xn[1], xn[2], pn[0], pn[1], pn[2]);
+  if (commit && turning && v0 < T(0)) {
+    T rt = gl_min<T>(r0, r1);
+    if (sc.options & kOptTurningRadius) {
+      T tlo, thi;
+      turning_root<T>(sc, K, h, y0[0], y0[1], y0[2], v0, v1, dir, tlo, thi, rt);
+    }
+    note_turning_radius<T>(R, rt);
+  }
+  R.turnings += (commit && turning) ? 1u : 0u;
+  R.crossings += (commit && zcross) ? 1u : 0u;
+  int32_t wind = commit ? winding_step<T>(y0[0], y0[1], xn[0], xn[1]) : 0;
+
+  // The state the attempt moves to: the step's end, or the localized event.
+  T ox = xn[0], oy = xn[1], oz = xn[2];
+  T ocx = cn[0], ocy = cn[1], ocz = cn[2];
+  T opx = pn[0], opy = pn[1], opz = pn[2];
+  T orr = r1, add_t = dtinc, add_lam = h;
+  if (terminal) {
+    // An accepted attempt that ends the ray on its earliest event.
     uint32_t mask = 0;
 #pragma unroll
     for (int i = 0; i < 4; ++i)
@@ -1025,8 +1061,8 @@ GL_HD void step(const SceneT<T>& sc, RayT<T>& R, T omega, bool reproject = false
       thz = T(thd);
       best = T(thd);
     }
-    kahan_add<T>(R.t, R.ct, dtl);
-    kahan_add<T>(R.lam, R.clam, best * h);
+    add_t = dtl;
+    add_lam = best * h;
     Deriv<T> dl;
     Geom<T> gl_;
     bool okl = eval<T>(sc, ys[0], ys[1], ys[2], ys[3], ys[4], ys[5], R.E, dl, gl_);
@@ -1044,7 +1080,7 @@ GL_HD void step(const SceneT<T>& sc, RayT<T>& R, T omega, bool reproject = false
       }
     }
     if (zcross && thz < best) R.crossings += 1;
-    update_winding<T>(R, y0[0], y0[1], ys[0], ys[1]);
+    wind = winding_step<T>(y0[0], y0[1], ys[0], ys[1]);
     if (okl)
       update_drift<T>(sc, R, gl_, dl, ys[0], ys[1], ys[2], ys[3], ys[4], ys[5]);
     else
@@ -1052,52 +1088,56 @@ GL_HD void step(const SceneT<T>& sc, RayT<T>& R, T omega, bool reproject = false
     const uint32_t terms[4] = {kTermSingularity, kTermHorizon, kTermSurface, kTermEscape};
     R.term = terms[kind];
     R.candidates = mask;
-    R.x = ys[0];
-    R.y = ys[1];
-    R.z = ys[2];
-    R.cx = cl[0];
-    R.cy = cl[1];
-    R.cz = cl[2];
-    R.px = ys[3];
-    R.py = ys[4];
-    R.pz = ys[5];
-    R.r = gl_.r;
-    return;
+    ox = ys[0];
+    oy = ys[1];
+    oz = ys[2];
+    ocx = cl[0];
+    ocy = cl[1];
+    ocz = cl[2];
+    opx = ys[3];
+    opy = ys[4];
+    opz = ys[5];
+    orr = gl_.r;
   }
 
-  // Non-terminal commit.
-  if (turning) {
-    R.turnings += 1;
-    if (v0 < T(0)) {
-      T rt = gl_min<T>(r0, r1);
-      if (sc.options & kOptTurningRadius) {
-        T tlo, thi;
-        turning_root<T>(sc, K, h, y0[0], y0[1], y0[2], v0, v1, dir, tlo, thi, rt);
-      }
-      note_turning_radius<T>(R, rt);
-    }
+  // Commit by selects: an accepted attempt moves the ray (a rejected or
+  // failed one keeps it and takes h_rej).
+  R.winding += wind;
+  {
+    T t = R.t, ct = R.ct, lam = R.lam, clam = R.clam;
+    kahan_add<T>(t, ct, add_t);
+    kahan_add<T>(lam, clam, add_lam);
+    R.t = accept ? t : R.t;
+    R.ct = accept ? ct : R.ct;
+    R.lam = accept ? lam : R.lam;
+    R.clam = accept ? clam : R.clam;
   }
-  if (zcross) R.crossings += 1;
-  kahan_add<T>(R.t, R.ct, dtinc);
-  kahan_add<T>(R.lam, R.clam, h);
-  R.x = xn[0];
-  R.y = xn[1];
-  R.z = xn[2];
-  R.cx = cn[0];
-  R.cy = cn[1];
-  R.cz = cn[2];
-  R.px = pn[0];
-  R.py = pn[1];
-  R.pz = pn[2];
+  R.x = accept ? ox : R.x;
+  R.y = accept ? oy : R.y;
+  R.z = accept ? oz : R.z;
+  R.cx = accept ? ocx : R.cx;
+  R.cy = accept ? ocy : R.cy;
+  R.cz = accept ? ocz : R.cz;
+  R.px = accept ? opx : R.px;
+  R.py = accept ? opy : R.py;
+  R.pz = accept ? opz : R.pz;
+  R.r = accept ? orr : R.r;
 #pragma unroll
-  for (int c = 0; c < 7; ++c) R.k[c] = K[6][c];
-  R.r = r1;
-  R.rdot = rd1;
-  update_winding<T>(R, y0[0], y0[1], xn[0], xn[1]);
-  if (reproject || drift_now)
-    update_drift<T>(sc, R, g, d, xn[0], xn[1], xn[2], pn[0], pn[1], pn[2]);
-  R.armed = arm_events<T>(sc, r1, z1, R.armed);
-  if (reproject) {
+  for (int c = 0; c < 7; ++c) R.k[c] = commit ? K[6][c] : R.k[c];
+  R.rdot = commit ? rd1 : R.rdot;
+  R.armed = commit ? arm_events<T>(sc, r1, z1, R.armed) : R.armed;
+  // Step control of policy.rs:160-173 (safety 0.9, factor in [0.2, 5] by
+  // default: the scene's safety, min_factor, max_factor) with
+  // no growth on the step right after a rejection (Hairer, Norsett & Wanner,
+  // Solving ODEs I, II.4), which breaks the accept/reject alternation of
+  // rays falling inward.
+  {
+    T f = en == T(0) ? sc.max_factor : pw;
+    f = gl_clamp<T>(f, sc.min_factor, after_reject ? T(1) : sc.max_factor);
+    const T h_acc = dir * gl_clamp<T>(hmag * f, sc.h_min, step_cap<T>(sc, r1));
+    R.h = commit ? h_acc : (reject ? h_rej : R.h);
+  }
+  if (commit && reproject) {
     reproject_null<T>(sc, R, R.px, R.py, R.pz);
     // The radial rate the next step compares against stays the committed
     // one: close to a turning point the radial velocity is tiny, and the
@@ -1107,18 +1147,9 @@ GL_HD void step(const SceneT<T>& sc, RayT<T>& R, T omega, bool reproject = false
     if (!refresh_derivative<T>(sc, R)) {
       R.term = kTermFailure;
       R.flags |= kFlagNonFinite;
-      return;
     }
     R.rdot = rdot_committed;
   }
-  // Step control of policy.rs:160-173 (safety 0.9, factor in [0.2, 5] by
-  // default: the scene's safety, min_factor, max_factor) with
-  // no growth on the step right after a rejection (Hairer, Norsett & Wanner,
-  // Solving ODEs I, II.4), which breaks the accept/reject alternation of
-  // rays falling inward.
-  T f = en == T(0) ? sc.max_factor : sc.safety * pow_m02<T>(en);
-  f = gl_clamp<T>(f, sc.min_factor, after_reject ? T(1) : sc.max_factor);
-  R.h = dir * gl_clamp<T>(hmag * f, sc.h_min, step_cap<T>(sc, R.r));
 }
 
 }  // namespace gl
