@@ -680,13 +680,18 @@ __device__ __forceinline__ void trace_f32(const SceneT<float>& sc, RayT<float>& 
                                           const DeferBuf& df, uint32_t fs, uint32_t idx,
                                           uint32_t& it, bool& deferred) {
   bool may_defer = (sc.options & kOptF64Horizon) != 0u;
+  // the re-projection clock, it mod reproject_interval, kept by increments
+  // (reproject_now's test without a division per attempt)
+  const uint32_t period = sc.reproject_interval;
+  uint32_t phase = period != 0u ? it % period : 1u;
   for (;;) {
     const bool live = !deferred && R.term == kTermActive;
     if (!__any(live)) break;
     if (live) {
       if (!(may_defer && R.r < sc.r_defer)) {
-        step<float>(sc, R, omega, reproject_now(sc, it), drift_now(it));
+        step<float>(sc, R, omega, period != 0u && phase == 0u, drift_now(it));
         ++it;
+        phase = phase + 1u == period ? 0u : phase + 1u;
       } else if (try_defer(df, R, fs, idx)) {
         deferred = true;
       } else {

@@ -345,7 +345,15 @@ template <class T>
 GL_HD T pow_m02(T v);  // v^-0.2 for the step controller
 template <>
 GL_HD float pow_m02<float>(float v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  // v_log_f32 / v_exp_f32 without the library's rescaling of denormal
+  // arguments: the same values for every normal v, and for a denormal v the
+  // factor (which the controller clamps to max_factor either way) is +inf
+  // instead of a large finite number.
+  return __builtin_amdgcn_exp2f(-0.2f * __builtin_amdgcn_logf(v));
+#else
   return exp2f(-0.2f * log2f(v));
+#endif
 }
 template <>
 GL_HD double pow_m02<double>(double v) {
