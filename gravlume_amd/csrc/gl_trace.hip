@@ -85,6 +85,16 @@ __device__ __forceinline__ uint32_t find_frame_scene(const FrameScene* frame, ui
 
 // kWaves: minimum waves per SIMD the register allocation must allow
 // (the second __launch_bounds__ argument); the autotuner picks it.
+//
+// Lane refill: a wavefront steps its rays in rounds of `interval` attempts
+// (rounded up to a multiple of 16); between rounds, lanes whose ray ended
+// write it, and when at least
+// `threshold` lanes are free they take the next items from the queue with
+// one atomicAdd per wavefront (ballot / popc / shfl). The stepping round
+// holds only the DP5 attempt and the f64-island test: a finished ray is
+// written at the start of the next round, so the output path's registers
+// and branches stay out of the stepping loop, whose exits are
+// wavefront-uniform (ballots).
 template <bool kSingle, int kWaves>
 __global__ void __launch_bounds__(256, kWaves) k_persistent(const SceneDev* __restrict__ scenes,
                                                     const FrameScene* __restrict__ frame,
@@ -95,8 +105,14 @@ __global__ void __launch_bounds__(256, kWaves) k_persistent(const SceneDev* __re
   const uint32_t lane = __lane_id();
   RayT<float> R;
   R.term = kTermActive;
-  bool active = false;
+  bool active = false;     // a ray in flight
+  bool done = false;       // a ray that ended and is not yet written
+  bool may_defer = false;  // the ray may still go to the f64 finisher
   uint32_t fs = 0, idx = 0;
+  // the ray's attempt clock (attempt number, and the same mod the
+  // re-projection interval), as in the plain and whole-tile kernels: a ray's
+  // arithmetic does not depend on the wavefront it was handed to
+  uint32_t it = 1, phase = 1;
   SceneT<float> sc;
   float omega = 0.0f;
   if (kSingle) {
@@ -110,6 +126,12 @@ __global__ void __launch_bounds__(256, kWaves) k_persistent(const SceneDev* __re
   uint32_t wave_steps = 0;  // wave attempt slots (lane slots / 64)
   bool drained = false;  // wave-uniform: the queue has no items left
   for (;;) {
+    if (done) {
+      const FrameScene& f = frame[fs];
+      const SceneDev& s = scenes[f.scene];
+      write_ray(R, f.out, idx, s, map, frame_source(f, s, idx), refine_sink(df), fs);
+      done = false;
+    }
     if (!drained) {
       unsigned long long free_mask = __ballot(!active);
       uint32_t nfree = __popcll(free_mask);
@@ -152,6 +174,9 @@ __global__ void __launch_bounds__(256, kWaves) k_persistent(const SceneDev* __re
             }
             if (have) {
               active = true;
+              may_defer = (sc.options & kOptF64Horizon) != 0u;
+              it = 1;
+              phase = sc.reproject_interval > 1u ? 1u : 0u;
               if (R.term != kTermActive) {
                 write_ray(R, f.out, idx, s, map, frame_source(f, s, idx), refine_sink(df), fs);
                 active = false;
@@ -165,27 +190,37 @@ __global__ void __launch_bounds__(256, kWaves) k_persistent(const SceneDev* __re
       if (drained) break;
       continue;
     }
-    for (uint32_t it = 0; it < interval; ++it) {
+    // Rounds of a multiple of 16 attempts between refills: every ray starts
+    // at a round boundary, so the attempt clocks of a wavefront's rays stay
+    // in phase modulo 16, and the drift sample (every 4th attempt) and the
+    // re-projection (every 16th, the default) run for all lanes on the same
+    // trips instead of some lane's on every trip.
+    const uint32_t round = (max(interval, 1u) + 15u) & ~15u;
+    for (uint32_t k = 0; k < round; ++k) {
       ++wave_steps;
       if (active) {
         ++lane_attempts;
-        if (wants_defer(sc, R) && try_defer(df, R, fs, idx)) {
+        bool deferred = false;
+        if (may_defer && R.r < sc.r_defer) {
+          deferred = try_defer(df, R, fs, idx);
+          may_defer = deferred;  // a full deferral buffer stays full: finish in f32
+        }
+        if (deferred) {
           active = false;
         } else {
-          // re-projection and drift sampling on the ray's own attempt clock,
-          // as in the plain and whole-tile kernels: a ray's arithmetic does
-          // not depend on the wavefront it was handed to
-          const uint32_t attempt = R.accepted + R.rejected + 1u;
-          step<float>(sc, R, omega, reproject_now(sc, attempt), drift_now(attempt));
+          const uint32_t period = sc.reproject_interval;
+          step<float>(sc, R, omega, period != 0u && phase == 0u, drift_now(it));
+          ++it;
+          phase = phase + 1u == period ? 0u : phase + 1u;
           if (R.term != kTermActive) {
-            const FrameScene& f = frame[fs];
-            const SceneDev& s = scenes[f.scene];
-            write_ray(R, f.out, idx, s, map, frame_source(f, s, idx), refine_sink(df), fs);
             active = false;
+            done = true;
           }
         }
       }
-      if (!drained && __popcll(__ballot(!active)) >= threshold) break;
+      if ((k & 15u) == 15u) {
+        if (!drained && __popcll(__ballot(!active)) >= threshold) break;
+      }
       if (drained && __ballot(active) == 0ull) break;
     }
   }

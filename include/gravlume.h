@@ -352,7 +352,10 @@ gl_status gl_trace_surface_footprint(gl_trace_plan* plan, uint32_t scene, const 
 typedef struct gl_persistent_config {
   uint32_t block_size;       /* lanes per workgroup: 64, 128, 192 or 256 */
   uint32_t blocks_per_cu;    /* resident workgroups per CU (capped by occupancy) */
-  uint32_t refill_interval;  /* DP5 attempts between refill checks */
+  uint32_t refill_interval;  /* DP5 attempts between refill checks, rounded
+                                up to a multiple of 16 (so the rays of a
+                                wavefront keep their drift and re-projection
+                                clocks in phase) */
   uint32_t refill_threshold; /* free lanes that trigger a lane refill, 1..63;
                                 64 = whole-tile mode: a wavefront takes one
                                 8x8 tile at a time and finishes it first */
