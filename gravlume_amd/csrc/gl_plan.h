@@ -125,5 +125,6 @@ gl_status launch_tile_order(gl_trace_plan* p, const FrameScene* d_table, size_t 
                             hipStream_t st);
 MapView map_view(const gl_trace_plan* p);
 
-__global__ void k_queue_reset(QueueDev* q, unsigned long long total, unsigned int* defer_count);
+__global__ void k_queue_reset(QueueDev* q, unsigned long long total,
+                              unsigned long long batch_first, unsigned int* defer_count);
 }  // namespace gl

@@ -83,25 +83,25 @@ __device__ __forceinline__ uint32_t find_frame_scene(const FrameScene* frame, ui
   return lo;
 }
 
-// kWaves: minimum waves per SIMD the register allocation must allow
-// (the second __launch_bounds__ argument); the autotuner picks it.
-//
 // Lane refill: a wavefront steps its rays in rounds of `interval` attempts
 // (rounded up to a multiple of 16); between rounds, lanes whose ray ended
-// write it, and when at least
-// `threshold` lanes are free they take the next items from the queue with
-// one atomicAdd per wavefront (ballot / popc / shfl). The stepping round
-// holds only the DP5 attempt and the f64-island test: a finished ray is
-// written at the start of the next round, so the output path's registers
-// and branches stay out of the stepping loop, whose exits are
-// wavefront-uniform (ballots).
-template <bool kSingle, int kWaves>
-__global__ void __launch_bounds__(256, kWaves) k_persistent(const SceneDev* __restrict__ scenes,
-                                                    const FrameScene* __restrict__ frame,
-                                                    uint32_t n_frame, QueueDev* queue, MapView map,
-                                                    DeferBuf df, uint32_t interval,
-                                                    uint32_t threshold) {
-  const unsigned long long total = queue->total;
+// write it, and when at least `threshold` lanes are free they take the next
+// items (global item indices from the cursor up to `total`) with one
+// atomicAdd per wavefront (ballot / popc / shfl). The stepping round holds
+// only the DP5 attempt and the f64-island test: a finished ray is written at
+// the start of the next round, so the output path's registers and branches
+// stay out of the stepping loop, whose exits are wavefront-uniform
+// (ballots). For kSingle, sc and omega hold the one scene's constants;
+// kBatchOnly: every item is a geodesic-batch request (no image path).
+template <bool kSingle, bool kBatchOnly = false>
+__device__ __forceinline__ void refill_rounds(const SceneDev* __restrict__ scenes,
+                                              const FrameScene* __restrict__ frame,
+                                              uint32_t n_frame, unsigned long long* cursor,
+                                              unsigned long long total, MapView map, DeferBuf df,
+                                              uint32_t interval, uint32_t threshold,
+                                              SceneT<float>& sc, float& omega,
+                                              uint32_t& lane_attempts, uint32_t& refills,
+                                              uint32_t& wave_steps) {
   const uint32_t lane = __lane_id();
   RayT<float> R;
   R.term = kTermActive;
@@ -113,18 +113,7 @@ __global__ void __launch_bounds__(256, kWaves) k_persistent(const SceneDev* __re
   // re-projection interval), as in the plain and whole-tile kernels: a ray's
   // arithmetic does not depend on the wavefront it was handed to
   uint32_t it = 1, phase = 1;
-  SceneT<float> sc;
-  float omega = 0.0f;
-  if (kSingle) {
-    sc = scene_f32(scenes[frame[0].scene]);
-    if (frame[0].out.turning_radius) sc.options |= kOptTurningRadius;
-    omega = scenes[frame[0].scene].omega;
-    hold_in_vgprs(sc);
-  }
-  uint32_t lane_attempts = 0;  // per lane: DP5 attempts made
-  uint32_t refills = 0;
-  uint32_t wave_steps = 0;  // wave attempt slots (lane slots / 64)
-  bool drained = false;  // wave-uniform: the queue has no items left
+  bool drained = false;  // wave-uniform: the cursor has no items left
   for (;;) {
     if (done) {
       const FrameScene& f = frame[fs];
@@ -138,7 +127,7 @@ __global__ void __launch_bounds__(256, kWaves) k_persistent(const SceneDev* __re
       if (nfree >= threshold || nfree == 64u) {
         uint32_t leader = __ffsll((long long)free_mask) - 1;
         unsigned long long base = 0;
-        if (lane == leader) base = atomicAdd(&queue->next, (unsigned long long)nfree);
+        if (lane == leader) base = atomicAdd(cursor, (unsigned long long)nfree);
         base = __shfl(base, leader);
         refills += 1;
         if (base + nfree >= total) drained = true;
@@ -156,14 +145,14 @@ __global__ void __launch_bounds__(256, kWaves) k_persistent(const SceneDev* __re
             }
             uint32_t local = uint32_t(item - f.first_item);
             bool have = false;
-            if (f.states) {  // geodesic batch request (items beyond rays pad to 64)
+            if (kBatchOnly || f.states) {  // geodesic batch request (items beyond rays pad to 64)
               if (local < f.rays) {
                 idx = local;
                 init_state_ray(s, sc, f.states + 8 * size_t(local),
                                f.dirs[local] >= 0.0f ? 1.0f : -1.0f, R);
                 have = true;
               }
-            } else {
+            } else if (!kBatchOnly) {
               uint32_t px, py;
               image_pixel(local, s.tiles_x, px, py);
               if (px < s.width && py < s.height) {
@@ -224,6 +213,30 @@ __global__ void __launch_bounds__(256, kWaves) k_persistent(const SceneDev* __re
       if (drained && __ballot(active) == 0ull) break;
     }
   }
+}
+
+// kWaves: minimum waves per SIMD the register allocation must allow
+// (the second __launch_bounds__ argument); the autotuner picks it.
+template <bool kSingle, int kWaves>
+__global__ void __launch_bounds__(256, kWaves) k_persistent(const SceneDev* __restrict__ scenes,
+                                                    const FrameScene* __restrict__ frame,
+                                                    uint32_t n_frame, QueueDev* queue, MapView map,
+                                                    DeferBuf df, uint32_t interval,
+                                                    uint32_t threshold) {
+  const uint32_t lane = __lane_id();
+  SceneT<float> sc;
+  float omega = 0.0f;
+  if (kSingle) {
+    sc = scene_f32(scenes[frame[0].scene]);
+    if (frame[0].out.turning_radius) sc.options |= kOptTurningRadius;
+    omega = scenes[frame[0].scene].omega;
+    hold_in_vgprs(sc);
+  }
+  uint32_t lane_attempts = 0;  // per lane: DP5 attempts made
+  uint32_t refills = 0;
+  uint32_t wave_steps = 0;  // wave attempt slots (lane slots / 64)
+  refill_rounds<kSingle>(scenes, frame, n_frame, &queue->next, queue->total, map, df, interval,
+                         threshold, sc, omega, lane_attempts, refills, wave_steps);
   for (int off = 32; off > 0; off >>= 1) lane_attempts += __shfl_xor(lane_attempts, off);
   if (lane == 0) {
     atomicAdd(&queue->attempts, (unsigned long long)lane_attempts);
@@ -242,6 +255,9 @@ __global__ void __launch_bounds__(256, kWaves) k_persistent_tiles(
     const SceneDev* __restrict__ scenes, const FrameScene* __restrict__ frame, uint32_t n_frame,
     QueueDev* queue, MapView map, DeferBuf df, uint32_t /*interval*/, uint32_t /*threshold*/) {
   const unsigned long long total = queue->total;
+  // image tiles lie below batch_first; the multi-entry kernel refills lanes
+  // from the geodesic-batch requests beyond it once the tiles are out
+  const unsigned long long tile_end = kSingle ? total : queue->batch_first;
   const uint32_t lane = __lane_id();
   uint32_t lane_attempts = 0, tiles = 0, wave_steps = 0;
   bool cost_changed = false;
@@ -249,7 +265,7 @@ __global__ void __launch_bounds__(256, kWaves) k_persistent_tiles(
     unsigned long long base = 0;
     if (lane == 0) base = atomicAdd(&queue->next, 64ull);
     base = __shfl(base, 0);
-    if (base >= total) break;
+    if (base >= tile_end) break;
     ++tiles;
     const unsigned long long item = base + lane;
     // the single-scene variant looks its frame slice up per tile as well:
@@ -297,6 +313,15 @@ __global__ void __launch_bounds__(256, kWaves) k_persistent_tiles(
       s.tile_cost[tile] = longest;
       cost_changed = true;
     }
+  }
+  if (!kSingle && tile_end < total) {
+    // Geodesic-batch requests (step counts from tens to ~10^4 within one
+    // batch, so whole 64-request tiles would leave most lanes idle): lane
+    // refill in rounds of 16 attempts, refilling at 16 free lanes.
+    SceneT<float> bsc;
+    float bomega = 0.0f;
+    refill_rounds<false, true>(scenes, frame, n_frame, &queue->batch_next, total, map, df, 16u, 16u,
+                         bsc, bomega, lane_attempts, tiles, wave_steps);
   }
   for (int off = 32; off > 0; off >>= 1) lane_attempts += __shfl_xor(lane_attempts, off);
   if (lane == 0) {
@@ -719,12 +744,15 @@ __global__ void __launch_bounds__(64) k_batch_f64(const SceneDev* __restrict__ s
   }
 }
 
-__global__ void k_queue_reset(QueueDev* q, unsigned long long total, unsigned int* defer_count) {
+__global__ void k_queue_reset(QueueDev* q, unsigned long long total,
+                              unsigned long long batch_first, unsigned int* defer_count) {
   if (blockIdx.x != 0) return;
   if (defer_count) clear_counters(defer_count);  // deferral, refine and ordering counters
   if (threadIdx.x == 0) {
     q->next = 0;
     q->total = total;
+    q->batch_first = batch_first;
+    q->batch_next = batch_first;
     q->refills = 0;
     q->attempts = 0;
     q->slots = 0;
@@ -1055,16 +1083,28 @@ gl_status build_frame_table(gl_trace_plan* p, const uint32_t* scenes, size_t n_s
 
 // Frame table of a mixed work list: image scenes (all pixels, 8x8-tile
 // ordered) and geodesic batches (requests padded to a multiple of 64 so that
-// every 64-item chunk lies inside one entry).
+// every 64-item chunk lies inside one entry). The image entries come first,
+// in the list's order, then the batches: in whole-tile mode the image tiles
+// are handed out whole and the batch requests from *batch_first on lane by
+// lane. Which entry a ray belongs to changes none of its outputs.
 gl_status build_work_table(gl_trace_plan* p, const gl_work_item* items, size_t n,
                            std::vector<FrameScene>* table, unsigned long long* total_items,
-                           size_t* total_rays) {
+                           size_t* total_rays, unsigned long long* batch_first) {
   if (!items || n == 0) return fail(GL_ERR_INVALID_ARGUMENT, "no work items");
   table->resize(n);
   unsigned long long total = 0;
   size_t rays_sum = 0;
-  for (size_t i = 0; i < n; ++i) {
+  std::vector<size_t> order;
+  order.reserve(n);
+  for (size_t i = 0; i < n; ++i)
+    if (items[i].kind != GL_WORK_GEODESIC_BATCH) order.push_back(i);
+  for (size_t i = 0; i < n; ++i)
+    if (items[i].kind == GL_WORK_GEODESIC_BATCH) order.push_back(i);
+  *batch_first = ~0ull;
+  for (size_t k = 0; k < n; ++k) {
+    const size_t i = order[k];
     const gl_work_item& w = items[i];
+    if (w.kind == GL_WORK_GEODESIC_BATCH && *batch_first == ~0ull) *batch_first = total;
     if (w.scene >= p->scenes.size()) return fail(GL_ERR_INVALID_ARGUMENT, "scene index out of range");
     const SceneDev& sd = p->scenes[w.scene];
     size_t rays = 0;
@@ -1089,10 +1129,11 @@ gl_status build_work_table(gl_trace_plan* p, const gl_work_item* items, size_t n
       f.states = w.states8;
       f.dirs = w.directions;
     }
-    (*table)[i] = f;
+    (*table)[k] = f;
     total += cnt;
     rays_sum += rays;
   }
+  if (*batch_first == ~0ull) *batch_first = total;
   *total_items = total;
   *total_rays = rays_sum;
   return GL_OK;
@@ -1129,11 +1170,15 @@ const void* persistent_kernel(bool single, uint32_t waves, bool tiles) {
 // graph's own).
 gl_status launch_persistent(gl_trace_plan* p, const FrameScene* d_table, size_t n_table,
                             unsigned long long total, size_t total_rays, hipStream_t st,
-                            const gl_persistent_config& c, Workspace& w) {
+                            const gl_persistent_config& c, Workspace& w,
+                            unsigned long long batch_first) {
   const bool f64 = any_f64_horizon(p), refine = any_refine(p);
   gl_status s = ensure_workspace(w, total_rays, f64, refine);
   if (s) return s;
-  hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, st, p->d_queue, total, w.counters);
+  // the lane-by-lane batch phase belongs to the multi-entry whole-tile kernel
+  if (n_table == 1 || c.refill_threshold < 64u) batch_first = total;
+  hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, st, p->d_queue, total, batch_first,
+                     w.counters);
   GL_HIP(hipGetLastError());
   // Every workgroup must be resident at once: clamp the requested blocks
   // per CU to what the kernel's register budget allows.
@@ -1201,7 +1246,7 @@ gl_status persistent_frame(gl_trace_plan* p, const uint32_t* scenes, size_t n_sc
   if (s) return s;
   s = upload_frame(p, table, st);
   if (s) return s;
-  return launch_persistent(p, p->d_frame, table.size(), total, rays, st, p->pcfg, p->ws);
+  return launch_persistent(p, p->d_frame, table.size(), total, rays, st, p->pcfg, p->ws, total);
 }
 
 }  // namespace gl
@@ -1363,7 +1408,7 @@ gl_status gl_trace_encode(gl_trace_plan* p, const gl_tile_region* region,
   if (s) return s;
   s = upload_frame(p, {make_frame(region->scene, 0, sd.width * sd.height, *out)}, st);
   if (s) return s;
-  hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, st, p->d_queue, 0ull, p->ws.counters);
+  hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, st, p->d_queue, 0ull, 0ull, p->ws.counters);
   const DeferBuf df = workspace_view(p->ws, f64, refine);
   size_t blocks = size_t(region->tiles_x) * region->tiles_y;
   mark(p, 0, st);
@@ -1398,14 +1443,15 @@ gl_status gl_trace_encode_work(gl_trace_plan* p, const gl_work_item* items, size
                                void* stream) {
   if (!p) return fail(GL_ERR_INVALID_ARGUMENT, "plan is required");
   std::vector<FrameScene> table;
-  unsigned long long total = 0;
+  unsigned long long total = 0, batch_first = 0;
   size_t rays = 0;
-  gl_status s = build_work_table(p, items, n_items, &table, &total, &rays);
+  gl_status s = build_work_table(p, items, n_items, &table, &total, &rays, &batch_first);
   if (s) return s;
   hipStream_t st = (hipStream_t)stream;
   s = upload_frame(p, table, st);
   if (s) return s;
-  return launch_persistent(p, p->d_frame, table.size(), total, rays, st, p->pcfg, p->ws);
+  return launch_persistent(p, p->d_frame, table.size(), total, rays, st, p->pcfg, p->ws,
+                           batch_first);
 }
 
 gl_status gl_geodesic_batch_trace_persistent(gl_trace_plan* p, uint32_t scene,
@@ -1430,7 +1476,7 @@ gl_status gl_geodesic_batch_trace_persistent(gl_trace_plan* p, uint32_t scene,
   f.dirs = directions;
   s = upload_frame(p, {f}, st);
   if (s) return s;
-  return launch_persistent(p, p->d_frame, 1, n, n, st, c, p->ws);
+  return launch_persistent(p, p->d_frame, 1, n, n, st, c, p->ws, n);
 }
 
 
@@ -1453,7 +1499,7 @@ gl_status gl_geodesic_batch_trace(gl_trace_plan* p, uint32_t scene, const double
   f.dirs = directions;
   s = upload_frame(p, {f}, st);
   if (s) return s;
-  hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, st, p->d_queue, 0ull, p->ws.counters);
+  hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, st, p->d_queue, 0ull, 0ull, p->ws.counters);
   const DeferBuf df = workspace_view(p->ws, f64, refine);
   mark(p, 0, st);
   hipLaunchKernelGGL(k_geodesic_batch, dim3(uint32_t((n + 63) / 64)), dim3(64), 0, st,
@@ -1581,7 +1627,7 @@ gl_status gl_geodesic_batch_trace_f64(gl_trace_plan* p, uint32_t scene, const do
   f.dirs = directions;
   s = upload_frame(p, {f}, st);
   if (s) return s;
-  hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, st, p->d_queue, 0ull, p->ws.counters);
+  hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, st, p->d_queue, 0ull, 0ull, p->ws.counters);
   mark(p, 0, st);
   hipLaunchKernelGGL(k_batch_f64, dim3(std::max(1, p->cu_count * 4)), dim3(64), 0, st,
                      p->d_scenes, p->d_frame, map_view(p), p->ws.counters + kBatchCursor,
@@ -1668,7 +1714,7 @@ gl_status gl_frame_graph_create(gl_trace_plan* p, const uint32_t* scenes, size_t
     return hip_fail(e, "hipStreamBeginCapture");
   }
   p->capturing = true;
-  s = launch_persistent(p, g->d_table, table.size(), total, rays, st, p->pcfg, g->ws);
+  s = launch_persistent(p, g->d_table, table.size(), total, rays, st, p->pcfg, g->ws, total);
   p->capturing = false;
   hipGraph_t graph = nullptr;
   e = hipStreamEndCapture(st, &graph);

@@ -81,8 +81,14 @@ struct alignas(64) QueueDev {
   unsigned long long slots;     // lane-attempt slots offered
   unsigned long long aux;       // second cursor (the accelerated sky trace's nodes)
   unsigned long long order_dirty;  // a whole-tile launch saw a tile cost change
-  unsigned long long pad;
+  // Whole-tile mode over a mixed work list: items from batch_first on are
+  // geodesic-batch requests, claimed lane by lane (from batch_next) once the
+  // image tiles are handed out; batch_first = total when there are none.
+  unsigned long long batch_first;
+  unsigned long long batch_next;
+  unsigned long long pad[7];
 };
+static_assert(sizeof(QueueDev) == 128, "QueueDev is two 64-byte lines");
 
 // Output planes as the kernels see them (see gl_ray_outputs in gravlume.h).
 struct OutPlanes {
