@@ -298,7 +298,12 @@ gl_status gl_trace_encode_persistent(gl_trace_plan* plan, const uint32_t* scenes
 
 /* Mixed work list for one persistent launch: whole image scenes and
  * geodesic batches of any scenes of the plan, all drawn from the same work
- * queue (uneven multi-scene batches in one launch). */
+ * queue (uneven multi-scene batches in one launch). In whole-tile mode
+ * (refill_threshold 64) with more than one item, the image scenes' 8x8 tiles
+ * are handed out whole first and the batch requests after them one lane at a
+ * time (lane refill in rounds of 16 attempts at 16 free lanes): requests of
+ * one batch differ in length by orders of magnitude, pixels of one tile do
+ * not. Each ray's outputs are the same in every mode. */
 enum { GL_WORK_IMAGE = 0u, GL_WORK_GEODESIC_BATCH = 1u };
 typedef struct gl_work_item {
   uint32_t scene;
