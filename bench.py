@@ -124,6 +124,25 @@ def cpu_baseline(scene, gpu_host, seconds, accuracy_rays, seed=20261203):
                       f"caps steps at 0.25 r, up to {scene.maximum_step:g} M), {elapsed:.1f} s on "
                       f"{cores} threads",
             "host": {"nproc": os.cpu_count(), "cpu_model": _cpu_model()}}
+    # like for like: the same timed sample kind with the GPU trace's step limits
+    # (initial step, 256 M maximum, |h| <= 0.25 r) instead of the reference
+    # policy's 0.5 M cap
+    h0 = scene.initial_step * scene.observer_radius
+    gpol = O.gpu_step_policy(scene.rtol, scene.atol, h0, scene.maximum_step,
+                             scene.maximum_step_per_radius)
+    done2, t2 = 0, time.perf_counter()
+    while time.perf_counter() - t2 < seconds / 2:
+        idx = rng.integers(0, scene.rays, 1024)
+        sxy = O.sight_planes(obs, idx % scene.width, idx // scene.width)
+        O.observation_trace_ordered(obs, gpol, ev, sxy, em, threads=cores)
+        done2 += len(idx)
+    el2 = time.perf_counter() - t2
+    base["gpu_step_policy"] = {
+        "rays_per_s": done2 / el2, "cores": cores,
+        "sample": f"{done2} seeded pixels, f64 DP5 rtol {scene.rtol:g} atol {scene.atol:g} with the "
+                  f"GPU trace's step limits (initial {h0:g} M, at most "
+                  f"{scene.maximum_step:g} M and {scene.maximum_step_per_radius:g} r), "
+                  f"{el2:.1f} s on {cores} threads"}
     # the small CPU configuration in full: 64x64, a = 0.5, Q = 0.3, rtol 1e-10
     from gravlume_amd import scene as S
     small = S.small_cpu_scene()

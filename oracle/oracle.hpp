@@ -105,6 +105,11 @@ struct Policy {
   uint32_t max_rejects;
   double event_affine_tol, event_tie_tol, arming_band, guard_d_over_m4;
   int id;  // 0 regular-v1, 1 strict-v1, 2 custom
+  // Not in the reference policy (0 there): when positive, a step is also
+  // capped at this multiple of the oblate radius of the state it starts from
+  // (the rejected step's start, or the accepted step's end), the GPU trace's
+  // step cap, for a CPU baseline that takes the GPU's step sequence.
+  double maximum_step_per_radius;
 };
 
 // EventConfiguration (events.rs:17-105) plus the horizon offset used by the

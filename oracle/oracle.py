@@ -38,7 +38,8 @@ class Policy(C.Structure):
                 ("safety", C.c_double), ("min_factor", C.c_double), ("max_factor", C.c_double),
                 ("max_accepted", C.c_uint64), ("max_rejects", C.c_uint32),
                 ("event_affine_tol", C.c_double), ("event_tie_tol", C.c_double),
-                ("arming_band", C.c_double), ("guard_d_over_m4", C.c_double), ("id", C.c_int)]
+                ("arming_band", C.c_double), ("guard_d_over_m4", C.c_double), ("id", C.c_int),
+                ("maximum_step_per_radius", C.c_double)]
 
 
 class Events(C.Structure):
@@ -180,6 +181,17 @@ def scaled_policy(base: Policy, divisor: float) -> Policy:
 def config_policy(rtol: float, atol: float, maximum_step: float = 0.5) -> Policy:
     return policy("regular", pos_rtol=rtol, pos_atol=atol, mom_rtol=rtol, mom_atol=atol,
                   maximum_step=maximum_step)
+
+
+def gpu_step_policy(rtol: float, atol: float, initial_step: float, maximum_step: float,
+                    maximum_step_per_radius: float) -> Policy:
+    """The reference policy at the workload's tolerances with the GPU trace's
+    step limits: its initial and maximum step and the relative cap
+    |h| <= maximum_step_per_radius * r (not part of the reference policy; for
+    a CPU baseline that takes the GPU's step sequence)."""
+    return policy("regular", pos_rtol=rtol, pos_atol=atol, mom_rtol=rtol, mom_atol=atol,
+                  initial_step=initial_step, maximum_step=maximum_step,
+                  maximum_step_per_radius=maximum_step_per_radius)
 
 
 def spacetime(mass=1.0, spin=0.0, charge=0.0, chart=1.0) -> Spacetime:

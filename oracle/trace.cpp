@@ -130,11 +130,26 @@ double error_norm(const Policy& p, const S8& s, const S8& e, const S8& err) {
   return m;
 }
 
-// ReferencePolicy::next_step_magnitude (policy.rs:160-173).
-double next_step(const Policy& p, double cur, double en, bool accepted) {
+// ReferencePolicy::next_step_magnitude (policy.rs:160-173); `cap` is the
+// policy's maximum step (the reference's), or less under the optional
+// relative cap (step_cap).
+double next_step(const Policy& p, double cur, double en, bool accepted, double cap) {
   double maxf = accepted ? p.max_factor : 1.0;
   double f = en == 0.0 ? maxf : std::clamp(p.safety * std::pow(en, -0.2), p.min_factor, maxf);
-  return std::clamp(cur * f, p.minimum_step, p.maximum_step);
+  return std::clamp(cur * f, p.minimum_step, cap);
+}
+
+// The maximum step from state y: the policy's, or with a positive
+// maximum_step_per_radius also that multiple of y's oblate radius (never
+// below the minimum step).
+double step_cap(const Policy& p, const Spacetime& st, const S8& y) {
+  double cap = p.maximum_step;
+  if (p.maximum_step_per_radius > 0.0) {
+    Geometry g;
+    if (geometry(st, &y[1], &g) == 0 && std::isfinite(g.radius))
+      cap = std::fmin(cap, p.maximum_step_per_radius * g.radius);
+  }
+  return std::fmax(cap, p.minimum_step);
 }
 
 enum Kind { kSing = 0, kHor = 1, kSurf = 2, kEsc = 3 };
@@ -252,7 +267,7 @@ class Execution {
         ++consecutive_rejects_;
         if (consecutive_rejects_ >= pol_.max_rejects) return finish(state_, kTermRejectExhaustion);
         if (std::fabs(step_) <= pol_.minimum_step) return finish_failure(kFailMinimumStep);
-        step_ = sign_ * next_step(pol_, std::fabs(step_), en, false);
+        step_ = sign_ * next_step(pol_, std::fabs(step_), en, false, step_cap(pol_, st_, state_));
         continue;
       }
       ++accepted_;
@@ -282,7 +297,7 @@ class Execution {
         if (!armed_[k] && value(k, state_, &v) == 0 && clears_band(k, v, pol_.arming_band))
           armed_[k] = true;
       }
-      step_ = sign_ * next_step(pol_, std::fabs(step_), en, true);
+      step_ = sign_ * next_step(pol_, std::fabs(step_), en, true, step_cap(pol_, st_, state_));
     }
   }
 
@@ -605,6 +620,7 @@ Policy regular_v1() {
   p.arming_band = 1.28e-9;
   p.guard_d_over_m4 = std::ldexp(1.0, -40);
   p.id = 0;
+  p.maximum_step_per_radius = 0.0;
   return p;
 }
 
