@@ -431,13 +431,17 @@ def test_capture_certificate_on_the_default_kerr_view(gl, oracle):
 
 
 @pytest.mark.parametrize("mode", [dict(refill_threshold=16, waves_per_simd=2),
-                                  dict(refill_threshold=64, waves_per_simd=2)],
-                         ids=["lane-refill", "whole-tile"])
+                                  dict(refill_threshold=64, waves_per_simd=2),
+                                  dict(refill_threshold=64, waves_per_simd=3)],
+                         ids=["lane-refill", "whole-tile", "whole-tile-3"])
 def test_mixed_work_list_matches_separate_launches(gl, mode):
     """One persistent launch over image scenes of uneven size plus, per scene,
     10% extra rays seeded next to its shadow edge as geodesic-batch requests
     (the uneven multi-scene workload in miniature) agrees with tracing each
-    image and each batch in its own plain launch."""
+    image and each batch in its own plain launch. In whole-tile mode the
+    launch hands out the image tiles whole and then the batch requests lane
+    by lane (a second cursor): every request is traced once, bit for bit as
+    in its own launch."""
     from gravlume_amd import scene as S
     import torch
     scenes = S.uneven_scenes(count=6, seed=11)
