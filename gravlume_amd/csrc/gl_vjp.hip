@@ -76,14 +76,23 @@ __global__ void __launch_bounds__(64) k_vjp_forward(const SceneDev* __restrict__
   RayT<float> R;
   init_image_ray(s, sc, px, py, R);
   bool overflow = false, island = false;
-  for (uint32_t it = 1; R.term == kTermActive; ++it) {
+  // the re-projection clock by increments, and a wavefront-uniform loop exit
+  // (a lane whose ray ended idles masked), as in the trace's stepping loop
+  const uint32_t period = sc.reproject_interval;
+  uint32_t it = 1, phase = period > 1u ? 1u : 0u;
+  for (;;) {
+    const bool live = R.term == kTermActive;
+    if (!__any(live)) break;
+    if (!live) continue;
     // where the trace would hand the ray to its f64 finisher, this forward
     // goes on in f32: such a ray is not the trace's, and gets no gradient
     island = island || wants_defer(sc, R);
     const float pre[6] = {R.x - R.cx, R.y - R.cy, R.z - R.cz, R.px, R.py, R.pz};
     const float h = R.h;
     const uint32_t acc = R.accepted;
-    step<float>(sc, R, s.omega, reproject_now(sc, it), drift_now(it));
+    step<float>(sc, R, s.omega, period != 0u && phase == 0u, drift_now(it));
+    ++it;
+    phase = phase + 1u == period ? 0u : phase + 1u;
     if (R.accepted != acc) {
       if (acc < vb.cap) {
         vb.h[size_t(acc) * rays + idx] = h;
