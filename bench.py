@@ -609,8 +609,26 @@ def main():
         assert launches == args.steps, (launches, args.steps)
         return wall, main_ms, fin_ms
 
-    t_pers, k_pers, f_pers = timed(lambda: pipe.encode_persistent([target]))
+    # serial frames: each frame's f64 passes follow its trace on one stream
+    t_serial, k_serial, f_serial = timed(lambda: pipe.encode_persistent([target]))
     stats = pipe.queue_stats()
+    # frame sequence with the f64 overlap: frame N's f64 passes run on the
+    # plan's second stream beside frame N+1's trace, the frames alternating
+    # between two targets; every frame is complete when the timed region's
+    # closing synchronize returns (outputs bit-identical to serial frames)
+    target_b = pipe.create_target(0)
+    frames = (target, target_b)
+    count = [0]
+
+    def overlapped():
+        pipe.encode_persistent([frames[count[0] % 2]])
+        count[0] += 1
+
+    pipe.set_f64_overlap(True)
+    for _ in range(2):
+        overlapped()
+    t_pers, k_pers, f_pers = timed(overlapped)
+    pipe.set_f64_overlap(False)
     # host-inclusive rate: each frame's 68 B/ray of output planes copied to
     # pinned host memory after the trace (PCIe), frames back to back
     planes = ("status", "steps", "rejected", "observable", "travel_time", "affine", "drift",
@@ -664,12 +682,15 @@ def main():
                                "FOV 15deg, f32 DP5(4) rtol 1e-6 atol 1e-8, disk [r_ISCO, 30M], "
                                "escape r=1000M into a 2048x1024 seeded RGB map",
                    "rays_per_step": rays, "kernel": kernel_variant(tuned["config"])},
+        "serial_rays_per_s": rays * args.steps / t_serial,
         "plain_rays_per_s": plain_value,
         "host_inclusive_rays_per_s": rays * args.steps / t_pcie,
         "graph_rays_per_s": rays * args.steps / t_graph,
-        "persistent_over_plain": value / plain_value,
-        "kernel_ms": {"persistent": k_pers, "persistent_f64_finisher": f_pers, "plain": k_plain,
-                      "plain_f64_finisher": f_plain},
+        "persistent_over_plain": (rays * args.steps / t_serial) / plain_value,
+        "overlap_gain": t_serial / t_pers,
+        "kernel_ms": {"persistent": k_pers, "persistent_f64_finisher": f_pers,
+                      "persistent_serial": k_serial, "persistent_serial_f64_finisher": f_serial,
+                      "plain": k_plain, "plain_f64_finisher": f_plain},
         "plain_persistent_termination_agreement": agree_plain,
         "steps_per_ray": {"accepted_mean": float(host["steps"].mean()),
                           "rejected_mean": float(host["rejected"].mean()),
@@ -688,6 +709,7 @@ def main():
                   "footprint_and_vjp": "none (every pixel of fixed scenes)"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "frac_of_nonpacked_fp32": achieved / 78.6,
+                     "frac_serial_frames": flops / (k_serial * 1e-3) / 1e12 / peak,
                      "traffic": measured_traffic(),
                      "traffic_from": "profiles/k_persistent_tiles_1080p_traffic.md: rocprofv3 --pmc "
                                      "FETCH_SIZE and WRITE_SIZE of this kernel variant on this "

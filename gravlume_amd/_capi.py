@@ -140,6 +140,8 @@ SIGNATURES = {
     "gl_trace_encode_persistent": (C.c_int, [_P, _P, C.c_size_t, C.POINTER(RayOutputsC), _P]),
     "gl_trace_plan_set_persistent_config": (C.c_int, [_P, C.POINTER(PersistentConfig)]),
     "gl_trace_plan_get_persistent_config": (C.c_int, [_P, C.POINTER(PersistentConfig)]),
+    "gl_trace_plan_set_f64_overlap": (C.c_int, [_P, C.c_int]),
+    "gl_trace_plan_flush": (C.c_int, [_P, _P]),
     "gl_geodesic_batch_trace": (C.c_int, [_P, C.c_uint32, _P, _P, C.c_size_t,
                                           C.POINTER(RayOutputsC), _P]),
     "gl_geodesic_batch_trace_noalloc": (C.c_int, [_P, C.c_uint32, _P, _P, C.c_size_t,

@@ -434,6 +434,8 @@ gl_status gl_trace_encode_accelerated(gl_trace_plan* p, uint32_t scene, const gl
                 "the escape-direction stencil applies to scenes without a surface");
   if (!out->observable) return fail(GL_ERR_INVALID_ARGUMENT, "the stencil needs the observable plane");
   hipStream_t st = (hipStream_t)stream;
+  s = drain_side(p);  // the workspace below may be reallocated and is rewritten
+  if (s) return s;
   const uint32_t tiles_x = (sd->width + 7u) / 8u, tiles_y = (sd->height + 7u) / 8u;
   const uint32_t node_w = 2u * tiles_x + 1u, node_h = 2u * tiles_y + 1u;
   const size_t nodes = size_t(node_w) * node_h;
@@ -485,6 +487,8 @@ gl_status gl_trace_refine_shadow(gl_trace_plan* p, uint32_t scene, const gl_ray_
   if (!out->status || !out->rgba16f)
     return fail(GL_ERR_INVALID_ARGUMENT, "shadow refinement needs the status and RGBA16F planes");
   hipStream_t st = (hipStream_t)stream;
+  s = join_side(p, st);  // the f64 passes of an overlapped launch may still write the planes
+  if (s) return s;
   const size_t rays = size_t(sd->width) * sd->height;
   if (rays + 1 > p->edge_capacity) {
     if (p->d_edges) (void)hipFree(p->d_edges);
@@ -523,9 +527,11 @@ gl_status gl_trace_surface_footprint(gl_trace_plan* p, uint32_t scene, const uin
   if (sd.disk_in <= 0.0f) return fail(GL_ERR_INVALID_ARGUMENT, "the footprint needs a surface");
   if (n == 0) return GL_OK;
   hipStream_t st = (hipStream_t)stream;
+  gl_status s = join_side(p, st);
+  if (s) return s;
   gl_ray_outputs none{};
   none.count = rays;
-  gl_status s = upload_frame(p, {make_frame(scene, 0, uint32_t(rays), none)}, st);
+  s = upload_frame(p, {make_frame(scene, 0, uint32_t(rays), none)}, st);
   if (s) return s;
   const uint32_t blocks = uint32_t(std::min<size_t>((n + kFootprintGroups - 1) / kFootprintGroups,
                                                     size_t(std::max(1, p->cu_count)) * 16));

@@ -88,10 +88,37 @@ struct gl_trace_plan {
   size_t node_capacity = 0;
   uint32_t* d_edges = nullptr;  // [0] count, [1..] pixel indices
   size_t edge_capacity = 0;
+  // f64 overlap (gl_trace_plan_set_f64_overlap): the f64 passes of an eager
+  // persistent launch run on the plan's second stream `side`, beside the
+  // next launch's trace. Launches alternate between two workspaces (ws and
+  // ws_alt) and two copies of the frame table the f64 passes read; a
+  // launch's passes are joined into its caller's stream after the next
+  // launch's trace kernel, or by any other entry point (join_side) or
+  // gl_trace_plan_flush.
+  bool overlap = false;
+  hipStream_t side = nullptr;
+  gl::Workspace ws_alt;
+  FrameScene* d_frame_ov[2] = {nullptr, nullptr};
+  size_t frame_ov_capacity[2] = {0, 0};
+  std::vector<FrameScene> frame_ov_shadow[2];
+  hipEvent_t ev_main[2] = {nullptr, nullptr};
+  hipEvent_t ev_fin[2] = {nullptr, nullptr};
+  bool fin_recorded[2] = {false, false};  // ev_fin[k] recorded, not yet waited for by the host
+  bool fin_unjoined[2] = {false, false};  // ev_fin[k] not yet waited for by the caller's stream
+  int ov_parity = 0;
 };
 
 
 namespace gl {
+// Makes `st` wait for every f64 pass still running on the plan's second
+// stream (f64 overlap), so that work enqueued on `st` afterwards may reuse
+// the plan's workspaces and see those passes' outputs.
+gl_status join_side(gl_trace_plan* p, hipStream_t st);
+// The host waits until no f64 pass is in flight on the plan's second stream
+// (before an entry point reallocates or rewrites the workspaces, captures a
+// graph or destroys the plan).
+gl_status drain_side(gl_trace_plan* p);
+
 // Records timing event `which` (0/1 main kernel start/stop, 2/3 finisher)
 // of the current launch on `st` when the plan's timing is enabled.
 void plan_mark(gl_trace_plan* p, int which, hipStream_t st);

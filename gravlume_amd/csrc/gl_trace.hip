@@ -807,6 +807,7 @@ struct gl_frame_graph {
   hipGraphExec_t exec = nullptr;
   FrameScene* d_table = nullptr;
   gl::Workspace ws;  // the graph's own deferral / refine buffers and counters
+  gl_trace_plan* plan = nullptr;
 };
 
 namespace gl {
@@ -1170,11 +1171,42 @@ const void* persistent_kernel(bool single, uint32_t waves, bool tiles) {
 // graph's own).
 gl_status launch_persistent(gl_trace_plan* p, const FrameScene* d_table, size_t n_table,
                             unsigned long long total, size_t total_rays, hipStream_t st,
-                            const gl_persistent_config& c, Workspace& w,
+                            const gl_persistent_config& c, Workspace& w_in,
                             unsigned long long batch_first) {
   const bool f64 = any_f64_horizon(p), refine = any_refine(p);
+  // f64 overlap: an eager launch (the plan's workspace, not being captured)
+  // alternates between two workspaces and frame-table copies, and runs its
+  // f64 passes on the plan's second stream
+  const bool ov = p->overlap && p->side && !p->capturing && &w_in == &p->ws && (f64 || refine);
+  const int k = p->ov_parity;
+  if (ov && p->fin_recorded[k]) {
+    // launch N-2's f64 passes read this workspace and frame copy: the host
+    // waits for them before it may grow or rewrite either
+    GL_HIP(hipEventSynchronize(p->ev_fin[k]));
+    p->fin_recorded[k] = false;
+  }
+  if (!ov && &w_in == &p->ws) {
+    // a serial launch on the plan's workspace: no earlier pass may still
+    // read it on the second stream
+    gl_status d = drain_side(p);
+    if (d) return d;
+  }
+  Workspace& w = ov && k ? p->ws_alt : w_in;
   gl_status s = ensure_workspace(w, total_rays, f64, refine);
   if (s) return s;
+  const FrameScene* f64_table = d_table;
+  if (ov) {
+    if (p->frame_ov_capacity[k] < n_table) {
+      if (p->d_frame_ov[k]) GL_HIP(hipFree(p->d_frame_ov[k]));
+      p->d_frame_ov[k] = nullptr;
+      p->frame_ov_capacity[k] = 0;
+      GL_HIP(hipMalloc(&p->d_frame_ov[k], n_table * sizeof(FrameScene)));
+      p->frame_ov_capacity[k] = n_table;
+    }
+    GL_HIP(hipMemcpyAsync(p->d_frame_ov[k], d_table, n_table * sizeof(FrameScene),
+                          hipMemcpyDeviceToDevice, st));
+    f64_table = p->d_frame_ov[k];
+  }
   // the lane-by-lane batch phase belongs to the multi-entry whole-tile kernel
   if (n_table == 1 || c.refill_threshold < 64u) batch_first = total;
   hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, st, p->d_queue, total, batch_first,
@@ -1208,10 +1240,48 @@ gl_status launch_persistent(gl_trace_plan* p, const FrameScene* d_table, size_t 
   }
   GL_HIP(hipGetLastError());
   mark(p, 1, st);
-  s = launch_f64_passes(p, d_table, df, st);
-  if (s) return s;
+  if (ov) {
+    GL_HIP(hipEventRecord(p->ev_main[k], st));
+    GL_HIP(hipStreamWaitEvent(p->side, p->ev_main[k], 0));
+    s = launch_f64_passes(p, f64_table, df, p->side);
+    if (s) return s;
+    GL_HIP(hipEventRecord(p->ev_fin[k], p->side));
+    p->fin_recorded[k] = true;
+    p->fin_unjoined[k] = true;
+    // the previous launch's passes ran beside this trace: join them
+    if (p->fin_unjoined[1 - k]) {
+      GL_HIP(hipStreamWaitEvent(st, p->ev_fin[1 - k], 0));
+      p->fin_unjoined[1 - k] = false;
+    }
+    p->ov_parity = 1 - k;
+  } else {
+    s = launch_f64_passes(p, d_table, df, st);
+    if (s) return s;
+  }
   // the next whole-tile launch hands out this launch's longest tiles first
   if (tiles) return launch_tile_order(p, d_table, n_table, st);
+  return GL_OK;
+}
+
+// f64 overlap: `st` waits for the passes still on the plan's second stream.
+gl_status join_side(gl_trace_plan* p, hipStream_t st) {
+  for (int k = 0; k < 2; ++k)
+    if (p->fin_unjoined[k]) {
+      GL_HIP(hipStreamWaitEvent(st, p->ev_fin[k], 0));
+      p->fin_unjoined[k] = false;
+    }
+  return GL_OK;
+}
+
+// f64 overlap: the host waits until the plan's second stream has finished
+// every pass, so that an entry point that reallocates or rewrites the
+// workspaces, or captures a graph, sees none of them in flight.
+gl_status drain_side(gl_trace_plan* p) {
+  for (int k = 0; k < 2; ++k) {
+    if (p->fin_recorded[k]) GL_HIP(hipEventSynchronize(p->ev_fin[k]));
+    p->fin_recorded[k] = false;
+    p->fin_unjoined[k] = false;
+  }
   return GL_OK;
 }
 
@@ -1364,6 +1434,16 @@ gl_status gl_trace_plan_create(const gl_scene_desc* scenes, size_t n_scenes,
 
 gl_status gl_trace_plan_destroy(gl_trace_plan* p) {
   if (!p) return GL_OK;
+  if (p->side) {
+    (void)hipStreamSynchronize(p->side);
+    (void)hipStreamDestroy(p->side);
+    for (int k = 0; k < 2; ++k) {
+      if (p->ev_main[k]) (void)hipEventDestroy(p->ev_main[k]);
+      if (p->ev_fin[k]) (void)hipEventDestroy(p->ev_fin[k]);
+      if (p->d_frame_ov[k]) (void)hipFree(p->d_frame_ov[k]);
+    }
+    free_workspace(p->ws_alt);
+  }
   if (p->d_scenes) (void)hipFree(p->d_scenes);
   if (p->d_map) (void)hipFree(p->d_map);
   if (p->d_lut) (void)hipFree(p->d_lut);
@@ -1404,6 +1484,8 @@ gl_status gl_trace_encode(gl_trace_plan* p, const gl_tile_region* region,
   if (s) return s;
   hipStream_t st = (hipStream_t)stream;
   const bool f64 = scene_f64_horizon(sd), refine = scene_refine(sd);
+  s = drain_side(p);
+  if (s) return s;
   s = ensure_workspace(p->ws, size_t(region->tiles_x) * region->tiles_y * 64, f64, refine);
   if (s) return s;
   s = upload_frame(p, {make_frame(region->scene, 0, sd.width * sd.height, *out)}, st);
@@ -1423,6 +1505,29 @@ gl_status gl_trace_encode_persistent(gl_trace_plan* p, const uint32_t* scenes, s
                                      const gl_ray_outputs* outputs, void* stream) {
   if (!p) return fail(GL_ERR_INVALID_ARGUMENT, "plan is required");
   return persistent_frame(p, scenes, n_scenes, outputs, (hipStream_t)stream);
+}
+
+gl_status gl_trace_plan_set_f64_overlap(gl_trace_plan* p, int enable) {
+  if (!p) return fail(GL_ERR_INVALID_ARGUMENT, "plan is required");
+  if (!enable) {
+    gl_status s = drain_side(p);
+    p->overlap = false;
+    return s;
+  }
+  for (int k = 0; k < 2; ++k) {
+    if (!p->ev_main[k]) GL_HIP(hipEventCreateWithFlags(&p->ev_main[k], hipEventDisableTiming));
+    if (!p->ev_fin[k]) GL_HIP(hipEventCreateWithFlags(&p->ev_fin[k], hipEventDisableTiming));
+  }
+  // default priority; non-blocking so that it overlaps a caller on the
+  // legacy default stream
+  if (!p->side) GL_HIP(hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking));
+  p->overlap = true;
+  return GL_OK;
+}
+
+gl_status gl_trace_plan_flush(gl_trace_plan* p, void* stream) {
+  if (!p) return fail(GL_ERR_INVALID_ARGUMENT, "plan is required");
+  return join_side(p, (hipStream_t)stream);
 }
 
 gl_status gl_trace_plan_set_persistent_config(gl_trace_plan* p, const gl_persistent_config* c) {
@@ -1492,6 +1597,8 @@ gl_status gl_geodesic_batch_trace(gl_trace_plan* p, uint32_t scene, const double
   if (s) return s;
   hipStream_t st = (hipStream_t)stream;
   const bool f64 = scene_f64_horizon(p->scenes[scene]), refine = scene_refine(p->scenes[scene]);
+  s = drain_side(p);
+  if (s) return s;
   s = ensure_workspace(p->ws, n, f64, refine);
   if (s) return s;
   FrameScene f = make_frame(scene, 0, uint32_t(n), *out);
@@ -1599,6 +1706,8 @@ gl_status gl_geodesic_batch_trace_noalloc(gl_trace_plan* p, uint32_t scene, cons
   f.states = states8;
   f.dirs = directions;
   hipStream_t st = (hipStream_t)stream;
+  s = join_side(p, st);  // no allocation here: the stream waits, the host does not
+  if (s) return s;
   hipLaunchKernelGGL(k_noalloc_begin, dim3(1), dim3(64), 0, st, f, frame, counters);
   GL_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_geodesic_batch, dim3(uint32_t((n + 63) / 64)), dim3(64), 0, st,
@@ -1620,6 +1729,8 @@ gl_status gl_geodesic_batch_trace_f64(gl_trace_plan* p, uint32_t scene, const do
   gl_status s = check_outputs(out, n);
   if (s) return s;
   hipStream_t st = (hipStream_t)stream;
+  s = drain_side(p);
+  if (s) return s;
   s = ensure_workspace(p->ws, 1, false, false);
   if (s) return s;
   FrameScene f = make_frame(scene, 0, uint32_t(n), *out);
@@ -1692,7 +1803,10 @@ gl_status gl_frame_graph_create(gl_trace_plan* p, const uint32_t* scenes, size_t
   size_t rays = 0;
   gl_status s = build_frame_table(p, scenes, n_scenes, outputs, &table, &total, &rays);
   if (s) return s;
+  s = drain_side(p);  // nothing of the second stream in flight across the capture
+  if (s) return s;
   auto* g = new gl_frame_graph();
+  g->plan = p;
   // The graph's own workspace, allocated before capture: eager calls on the
   // plan may later grow or replace the plan's without affecting replays.
   s = ensure_workspace(g->ws, rays, any_f64_horizon(p), any_refine(p));
@@ -1739,6 +1853,8 @@ gl_status gl_frame_graph_create(gl_trace_plan* p, const uint32_t* scenes, size_t
 
 gl_status gl_frame_graph_launch(gl_frame_graph* g, void* stream) {
   if (!g || !g->exec) return fail(GL_ERR_INVALID_ARGUMENT, "graph is required");
+  gl_status s = join_side(g->plan, (hipStream_t)stream);
+  if (s) return s;
   GL_HIP(hipGraphLaunch(g->exec, (hipStream_t)stream));
   return GL_OK;
 }

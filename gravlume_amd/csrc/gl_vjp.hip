@@ -480,6 +480,11 @@ gl_status gl_vjp_forward(gl_trace_plan* p, uint32_t scene, double inclination,
   map.w = p->map_w;
   map.h = p->map_h;
   const OutPlanes planes = to_planes(*out);
+  s = join_side(p, hs_);
+  if (s) {
+    release(st);
+    return s;
+  }
   plan_mark(p, 0, hs_);
   hipLaunchKernelGGL(k_vjp_forward, dim3(st->tiles), dim3(64), 0, hs_, p->d_scenes, scene, planes,
                      map, vb);

@@ -268,6 +268,19 @@ class TracePipeline:
                                    waves_per_simd)
         _capi.check(self._lib.gl_trace_plan_set_persistent_config(self._handle, C.byref(c)))
 
+    def set_f64_overlap(self, enable: bool) -> None:
+        """f64 overlap for frame sequences (gl_trace_plan_set_f64_overlap): a
+        persistent launch's f64 passes run on the plan's second stream beside
+        the next launch's trace; its outputs are complete after the next
+        launch's trace kernel or after flush(). Consecutive launches must
+        write different targets."""
+        _capi.check(self._lib.gl_trace_plan_set_f64_overlap(self._handle, 1 if enable else 0))
+
+    def flush(self, stream=None) -> None:
+        """Makes `stream` wait for every f64 pass still in flight on the
+        plan's second stream (gl_trace_plan_flush)."""
+        _capi.check(self._lib.gl_trace_plan_flush(self._handle, _stream_handle(stream)))
+
     def persistent_config(self) -> dict:
         c = _capi.PersistentConfig()
         _capi.check(self._lib.gl_trace_plan_get_persistent_config(self._handle, C.byref(c)))

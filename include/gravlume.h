@@ -372,6 +372,24 @@ gl_status gl_trace_plan_set_persistent_config(gl_trace_plan* plan,
 gl_status gl_trace_plan_get_persistent_config(const gl_trace_plan* plan,
                                               gl_persistent_config* config);
 
+/* f64 overlap for frame sequences (off by default; no reference counterpart:
+ * the reference renders one frame at a time). When enabled, an eager
+ * persistent launch (gl_trace_encode_persistent, gl_trace_encode_work,
+ * gl_geodesic_batch_trace_persistent) of a plan
+ * whose scenes defer near-horizon steps or refine near-critical rays runs its
+ * f64 passes (near-horizon finisher, refine) on a second stream the plan
+ * owns, beside the next launch's trace. A launch's outputs are then complete
+ * on the caller's stream only after the NEXT launch's trace kernel, or after
+ * gl_trace_plan_flush on that stream; every other entry point, and disabling
+ * the mode, joins the outstanding passes first. Until then the launch's
+ * inputs and output planes must stay untouched, so two consecutive
+ * overlapped launches write different output planes (alternate two
+ * targets). Results are bit-identical to the serial launches. */
+gl_status gl_trace_plan_set_f64_overlap(gl_trace_plan* plan, int enable);
+/* Makes `stream` wait for every f64 pass still running on the plan's second
+ * stream (no host wait). */
+gl_status gl_trace_plan_flush(gl_trace_plan* plan, void* stream);
+
 /* Geodesic batch (GeodesicBatch::trace_ordered, batch.rs:31-43) on the GPU:
  * n requests of scene `scene`'s spacetime, policy and events, each an
  * initial state (t, x, y, z, p_t, p_x, p_y, p_z) in f64 on the device and an
