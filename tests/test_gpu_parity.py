@@ -563,6 +563,79 @@ def test_shadow_edge_requests_match_the_oracle(gl, oracle, refine):
             assert np.array_equal(h[key][idx][settled], A[key][settled]), key
 
 
+def test_uneven_workload_launch_matches_the_oracle(gl, oracle):
+    """The uneven multi-scene workload itself, as the benchmark builds it: the
+    256 scenes of seed 20261206 at their drawn sizes (1.03e8 pixels) and,
+    per scene, 10% of its pixel count as shadow-edge requests (seed
+    20261207 + scene index, 1.03e7 requests), all in one whole-tile
+    work-list launch. Eight seeded pixels and eight seeded requests of every
+    scene (2,048 of each) against the oracle at rtol 1e-10: the pooled pixel
+    sample with the frame tolerances above; on the requests every definite
+    class and event id equals the oracle's and every settled branch key is
+    exact, the f32 trace leaving the requests within its near-critical
+    margin Uncertain."""
+    from gravlume_amd import scene as S
+    import torch
+    seed = 20261206
+    scenes = S.uneven_scenes(count=256, seed=seed)
+    pipe = gl.TracePipeline(scenes)
+    items, keep = [], []
+    for i, s in enumerate(scenes):
+        img = pipe.create_target(i)
+        items.append(("image", i, img))
+        states, _ = S.shadow_edge_requests(s, 0.1, seed=seed + 1 + i)
+        d_states = torch.from_numpy(states).cuda()
+        d_dirs = -torch.ones(len(states), dtype=torch.float32, device="cuda")
+        bt = gl.TraceTarget(len(states))
+        items.append(("batch", i, d_states, d_dirs, bt))
+        keep.append((img, bt, states, d_states, d_dirs))
+    print("rays", sum(s.rays for s in scenes), "requests", sum(len(k[2]) for k in keep), flush=True)
+    pipe.encode_work(items)
+    torch.cuda.synchronize()
+    rng = np.random.default_rng(seed + 4)
+    hp, ap, hq, aq = [], [], [], []
+    for i, s in enumerate(scenes):
+        img, bt, states, _, _ = keep[i]
+        h = img.to_host()
+        assert np.all(h["termination"] > 0), i
+        idx = np.sort(rng.choice(s.rays, 8, replace=False))
+        hp.append({k: v[idx] for k, v in h.items()})
+        ap.append(_oracle_outcomes(oracle, s, idx, threads=16))
+        hb = bt.to_host()
+        assert np.all(hb["termination"] > 0), i
+        j = np.sort(rng.choice(len(states), 8, replace=False))
+        hq.append({k: v[j] for k, v in hb.items()})
+        st = oracle.spacetime(1.0, s.spin, s.charge, float(s.chart))
+        ev = oracle.events(s.escape_radius, s.disk_radii(), horizon_offset=s.horizon_offset)
+        A = oracle.outcomes_to_arrays(oracle.trace_ordered(
+            st, oracle.config_policy(1e-10, 1e-12), ev, states[j], -np.ones(8), threads=16))
+        F = oracle.outcomes_to_arrays(oracle.trace_ordered(
+            st, oracle.config_policy(1e-10 / 16, 1e-12 / 16), ev, states[j], -np.ones(8),
+            threads=16))
+        A["flagged"] = (A["termination"] != F["termination"]) | (A["candidates"] != F["candidates"])
+        aq.append(A)
+
+    def pool(parts):
+        return {k: np.concatenate([p[k] for p in parts]) for k in parts[0]}
+
+    H, A = pool(hp), pool(ap)
+    n = len(H["termination"])
+    print("pixels", _compare(H, A, np.arange(n), dict(TOL, uncertain=1.2e-2)),
+          "flagged", int(A["flagged"].sum()))
+    H, A = pool(hq), pool(aq)
+    ok = ~A["flagged"]
+    t = H["termination"]
+    definite = ok & (t != 6)
+    print("requests: flagged", int((~ok).sum()), "uncertain", int((ok & (t == 6)).sum()),
+          "definite", int(definite.sum()))
+    assert definite.sum() >= 1024
+    assert np.array_equal(t[definite], A["termination"][definite])
+    assert np.array_equal(H["candidates"][definite], A["candidates"][definite])
+    settled = definite & ~H["branch_unsettled"]
+    for key in ("radial_turnings", "equatorial_crossings", "azimuth_winding", "polar_side"):
+        assert np.array_equal(H[key][settled], A[key][settled]), key
+
+
 def test_default_view_produces_only_determinate_rays(gl):
     """The reference's default view at 80x45 (gpu_trace_tests.rs:356-378):
     every ray crosses the horizon or escapes, and both occur."""
