@@ -108,7 +108,7 @@ def cpu_baseline(scene, gpu_host, seconds, accuracy_rays, seed=20261203):
                         scene.width, scene.height, np.radians(scene.fov_deg))
     ev = O.events(scene.escape_radius, scene.disk_radii(), horizon_offset=scene.horizon_offset)
     em = O.Emitter()
-    rng = np.random.default_rng(seed)
+    rng = np.random.default_rng(seed + 1)
     # timed leg: the workload's own tolerance
     pol = O.config_policy(scene.rtol, scene.atol)
     done, t0 = 0, time.perf_counter()
@@ -125,11 +125,11 @@ def cpu_baseline(scene, gpu_host, seconds, accuracy_rays, seed=20261203):
                       f"{cores} threads",
             "host": {"nproc": os.cpu_count(), "cpu_model": _cpu_model()}}
     # like for like: the same timed sample kind with the GPU trace's step limits
-    # (initial step, 256 M maximum, |h| <= 0.25 r) instead of the reference
-    # policy's 0.5 M cap
+    # (initial step, 256 M maximum, |h| <= 0.25 r, growth cap) instead of the
+    # reference policy's 0.5 M cap and growth cap 5
     h0 = scene.initial_step * scene.observer_radius
     gpol = O.gpu_step_policy(scene.rtol, scene.atol, h0, scene.maximum_step,
-                             scene.maximum_step_per_radius)
+                             scene.maximum_step_per_radius, scene.step_max_factor)
     done2, t2 = 0, time.perf_counter()
     while time.perf_counter() - t2 < seconds / 2:
         idx = rng.integers(0, scene.rays, 1024)
@@ -141,7 +141,8 @@ def cpu_baseline(scene, gpu_host, seconds, accuracy_rays, seed=20261203):
         "rays_per_s": done2 / el2, "cores": cores,
         "sample": f"{done2} seeded pixels, f64 DP5 rtol {scene.rtol:g} atol {scene.atol:g} with the "
                   f"GPU trace's step limits (initial {h0:g} M, at most "
-                  f"{scene.maximum_step:g} M and {scene.maximum_step_per_radius:g} r), "
+                  f"{scene.maximum_step:g} M and {scene.maximum_step_per_radius:g} r, growth "
+                  f"at most {scene.step_max_factor:g}x per step), "
                   f"{el2:.1f} s on {cores} threads"}
     # the small CPU configuration in full: 64x64, a = 0.5, Q = 0.3, rtol 1e-10
     from gravlume_amd import scene as S
@@ -159,8 +160,10 @@ def cpu_baseline(scene, gpu_host, seconds, accuracy_rays, seed=20261203):
                                  "sample": "all 4096 pixels of the 64x64 a=0.5 Q=0.3 view, f64 "
                                            "DP5 rtol 1e-10 atol 1e-12"}
     # accuracy leg: f64 reference at rtol 1e-10, and its 1/16 refinement for
-    # the near-critical flag (termination changes between the two)
-    idx = rng.choice(scene.rays, accuracy_rays, replace=False)
+    # the near-critical flag (termination changes between the two), on a
+    # sample drawn from its own generator (the timed legs above draw as many
+    # batches as their time allows)
+    idx = np.random.default_rng(seed).choice(scene.rays, accuracy_rays, replace=False)
     sxy = O.sight_planes(obs, idx % scene.width, idx // scene.width)
     ref, _ = O.observation_trace_ordered(obs, O.config_policy(1e-10, 1e-12), ev, sxy, em,
                                          threads=cores)
@@ -703,7 +706,7 @@ def main():
             "not_representable": int(np.sum((host["termination"] == 7) & (host["radiance"][:, 0] < 0))),
             "model": f"{scene.emission} T_6 = {scene.six_temperature():.6g} K, I_6 = {scene.intensity_at_six}, vacuum",
         },
-        "seeds": {"escape_map": 20261203, "cpu_baseline_and_accuracy_sample": 20261203,
+        "seeds": {"escape_map": 20261203, "accuracy_sample": 20261203, "cpu_baseline_sample": 20261204,
                   "uneven_scenes": 20261206, "uneven_shadow_edge_rays": "20261207 + scene index",
                   "near_critical_offsets": 20261208, "near_critical_oracle_sample": 20261209,
                   "footprint_and_vjp": "none (every pixel of fixed scenes)"},

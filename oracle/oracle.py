@@ -184,14 +184,15 @@ def config_policy(rtol: float, atol: float, maximum_step: float = 0.5) -> Policy
 
 
 def gpu_step_policy(rtol: float, atol: float, initial_step: float, maximum_step: float,
-                    maximum_step_per_radius: float) -> Policy:
+                    maximum_step_per_radius: float, max_factor: float = 5.0) -> Policy:
     """The reference policy at the workload's tolerances with the GPU trace's
-    step limits: its initial and maximum step and the relative cap
-    |h| <= maximum_step_per_radius * r (not part of the reference policy; for
-    a CPU baseline that takes the GPU's step sequence)."""
+    step limits: its initial and maximum step, the relative cap
+    |h| <= maximum_step_per_radius * r (not part of the reference policy) and
+    the controller's growth cap max_factor (the reference's is 5); for a CPU
+    baseline that takes the GPU's step sequence."""
     return policy("regular", pos_rtol=rtol, pos_atol=atol, mom_rtol=rtol, mom_atol=atol,
                   initial_step=initial_step, maximum_step=maximum_step,
-                  maximum_step_per_radius=maximum_step_per_radius)
+                  maximum_step_per_radius=maximum_step_per_radius, max_factor=max_factor)
 
 
 def spacetime(mass=1.0, spin=0.0, charge=0.0, chart=1.0) -> Spacetime:
