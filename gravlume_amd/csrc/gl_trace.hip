@@ -1244,7 +1244,11 @@ gl_status launch_persistent(gl_trace_plan* p, const FrameScene* d_table, size_t 
     GL_HIP(hipEventRecord(p->ev_main[k], st));
     GL_HIP(hipStreamWaitEvent(p->side, p->ev_main[k], 0));
     s = launch_f64_passes(p, f64_table, df, p->side);
-    if (s) return s;
+    if (s) {
+      // nothing of a failed launch may stay in flight unrecorded
+      (void)hipStreamSynchronize(p->side);
+      return s;
+    }
     GL_HIP(hipEventRecord(p->ev_fin[k], p->side));
     p->fin_recorded[k] = true;
     p->fin_unjoined[k] = true;
@@ -1437,13 +1441,13 @@ gl_status gl_trace_plan_destroy(gl_trace_plan* p) {
   if (p->side) {
     (void)hipStreamSynchronize(p->side);
     (void)hipStreamDestroy(p->side);
-    for (int k = 0; k < 2; ++k) {
-      if (p->ev_main[k]) (void)hipEventDestroy(p->ev_main[k]);
-      if (p->ev_fin[k]) (void)hipEventDestroy(p->ev_fin[k]);
-      if (p->d_frame_ov[k]) (void)hipFree(p->d_frame_ov[k]);
-    }
-    free_workspace(p->ws_alt);
   }
+  for (int k = 0; k < 2; ++k) {
+    if (p->ev_main[k]) (void)hipEventDestroy(p->ev_main[k]);
+    if (p->ev_fin[k]) (void)hipEventDestroy(p->ev_fin[k]);
+    if (p->d_frame_ov[k]) (void)hipFree(p->d_frame_ov[k]);
+  }
+  free_workspace(p->ws_alt);
   if (p->d_scenes) (void)hipFree(p->d_scenes);
   if (p->d_map) (void)hipFree(p->d_map);
   if (p->d_lut) (void)hipFree(p->d_lut);
