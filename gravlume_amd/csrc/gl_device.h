@@ -246,26 +246,29 @@ __device__ void init_state_ray(const SceneDev& s, const SceneT<float>& sc, const
 }
 
 // ---------------------------------------------------------------------------
-// Surface radiance (f64, once per surface hit)
+// Surface radiance (once per surface hit): the intensity products and the
+// RGBA16F range tests in f64, the band lookup's logarithms and powers of two
+// in f32 (hardware log2 / exp2, about one ulp; a relative error near 1e-6 in
+// the radiance, below the binary16 texel's resolution of 2^-11).
 
 constexpr double kHalfMax = 65504.0;                        // largest finite binary16
 constexpr double kMinNormal32 = 1.1754943508222875e-38;     // 2^-126
 
-// log2 band fractions of a blackbody at T from the band table, linear in
-// log2 T between entries (blackbody_surface_preview.wgsl:10-33); false
-// outside the table or for a malformed entry.
-__device__ __forceinline__ bool band_log2_fractions(const float* lut, double t, double f[3]) {
-  if (lut == nullptr || !(t > 0.0) || !(t < 1.0e300)) return false;
-  const double c = (::log2(t) - GL_BLACKBODY_LUT_MIN_LOG2_K) * GL_BLACKBODY_LUT_PER_OCTAVE;
-  if (!(c >= 0.0 && c <= double(GL_BLACKBODY_LUT_ENTRIES - 1))) return false;
+// log2 band fractions of a blackbody at log2 T = lt from the band table,
+// linear in log2 T between entries (blackbody_surface_preview.wgsl:10-33);
+// false outside the table or for a malformed entry.
+__device__ __forceinline__ bool band_log2_fractions(const float* lut, float lt, float f[3]) {
+  if (lut == nullptr || !(lt == lt)) return false;
+  const float c = (lt - float(GL_BLACKBODY_LUT_MIN_LOG2_K)) * float(GL_BLACKBODY_LUT_PER_OCTAVE);
+  if (!(c >= 0.0f && c <= float(GL_BLACKBODY_LUT_ENTRIES - 1))) return false;
   const int i = min(int(c), GL_BLACKBODY_LUT_ENTRIES - 2);
-  const double w = c - double(i);
+  const float w = c - float(i);
   const float4 lo = reinterpret_cast<const float4*>(lut)[i];
   const float4 hi = reinterpret_cast<const float4*>(lut)[i + 1];
-  f[0] = double(lo.x) + w * (double(hi.x) - double(lo.x));
-  f[1] = double(lo.y) + w * (double(hi.y) - double(lo.y));
-  f[2] = double(lo.z) + w * (double(hi.z) - double(lo.z));
-  return f[0] <= 0.0 && f[1] <= 0.0 && f[2] <= 0.0;
+  f[0] = lo.x + w * (hi.x - lo.x);
+  f[1] = lo.y + w * (hi.y - lo.y);
+  f[2] = lo.z + w * (hi.z - lo.z);
+  return f[0] <= 0.0f && f[1] <= 0.0f && f[2] <= 0.0f;
 }
 
 // A positive radiance product resolved for RGBA16F: 0 below the f32 normal
@@ -274,6 +277,16 @@ __device__ __forceinline__ bool resolve_half(double v, double& out) {
   if (!(v <= kHalfMax)) return false;
   out = v < kMinNormal32 ? 0.0 : v;
   return true;
+}
+
+// v 2^f over f64's whole range: the power of two split as 2^n 2^(f - n),
+// n = floor(f), its fraction from the f32 hardware exp2 and 2^n applied
+// exactly (an f32 2^f alone would flush to zero below 2^-126, where
+// v 2^f can still be a normal value).
+__device__ __forceinline__ double scale_exp2(double v, float f) {
+  if (!(f > -1.0e9f)) return 0.0;  // a zero band fraction (log2 = -inf)
+  const float n = floorf(f);
+  return ldexp(v * double(__builtin_amdgcn_exp2f(f - n)), int(n));
 }
 
 struct Rgb {
@@ -301,16 +314,20 @@ __device__ __noinline__ Rgb surface_radiance(const SceneDev& s, float r, float g
     const float o = float(v + src);
     return Rgb{o, o, o};
   }
-  double fi[3], fs[3] = {0.0, 0.0, 0.0};
-  const double t_obs = s.temperature6_d / __builtin_sqrt(rr * __builtin_sqrt(rr)) * gd;
-  if (!band_log2_fractions(s.lut, t_obs, fi)) return bad;
-  if (src > 0.0 && !band_log2_fractions(s.lut, s.slab_temperature_d, fs)) return bad;
+  // log2 of the observed temperature, log2 T_6 - 3/4 log2(r/6) + log2 g
+  const float lt = __builtin_amdgcn_logf(float(s.temperature6_d)) -
+                   0.75f * __builtin_amdgcn_logf(float(rr)) + __builtin_amdgcn_logf(g);
+  float fi[3], fs[3] = {0.0f, 0.0f, 0.0f};
+  if (!band_log2_fractions(s.lut, lt, fi)) return bad;
+  if (src > 0.0 &&
+      !band_log2_fractions(s.lut, __builtin_amdgcn_logf(float(s.slab_temperature_d)), fs))
+    return bad;
   double out[3];
 #pragma unroll
   for (int b = 0; b < 3; ++b) {
     double in_b, src_b = 0.0;
-    if (!resolve_half(incoming * ::exp2(fi[b]), in_b)) return bad;
-    if (src > 0.0 && !resolve_half(src * ::exp2(fs[b]), src_b)) return bad;
+    if (!resolve_half(scale_exp2(incoming, fi[b]), in_b)) return bad;
+    if (src > 0.0 && !resolve_half(scale_exp2(src, fs[b]), src_b)) return bad;
     if (!(in_b + src_b <= kHalfMax)) return bad;
     out[b] = in_b + src_b;
   }
